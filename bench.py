@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Benchmark: MLUPS of the plain fp32 D3Q19 Allen-Cahn step on the
+150x150x107 ridge-impact case, its HBM roofline fraction, and a CPU baseline.
+
+Workload (the README construction of the reference, fp32): FEConfig(nx=150,
+ny=150, nz=107, rho_l=1, rho_g=1/828, sigma=0.01424, xi=4, tau_l=tau_g=0.53,
+theta_eq=162), AllenCahnSolver(dtype=float32, geo_amplification=1.5,
+geometric_wetting=True) with boundary_relax=0.3 and tau_h=1.0; ridge R*=1,
+R_d=22.5 with solid fraction; droplet R=22.5 at (75,75,80), u=(0,0,-0.05).
+Inputs are resident in HBM before the timed region; a "step" is one
+solver.step() over all 2,407,500 nodes (solid included).
+
+Prints one JSON line (rank 0).  Only one GPU is supported.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "pytorch-lbm_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = ("MLUPS + % HBM roofline, D3Q19 150×150×107; "
+          "sweep solves/hr + p99 latency")
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per lattice update (compulsory state traffic once per step):
+# f and g read+write (2*19*2*4 = 304), u read+write (24), phi*u_prev read+write
+# (24), phi write (4)  ->  356 B/LU for the whole fp32 step.
+B_STEP = 356
+# The collide kernel alone: f, g read+write (304), u read+write (24), phi read
+# (4), previous phi read (4; phi*u_prev is recomputed from it) -> 336 B/LU.
+B_COLLIDE = 336
+
+
+def cfg2(storage=None, dtype="float32"):
+    import torch
+    from geometry.substrate import create_substrate_with_fraction
+    from lbm.fe_ac_solver import AllenCahnSolver
+    from lbm.fe_config import FEConfig
+    from lbm.fe_droplet import create_fe_droplet_with_impact
+    cfg = FEConfig(nx=150, ny=150, nz=107, rho_l=1.0, rho_g=1.0 / 828.0, sigma=0.01424,
+                   xi=4.0, tau_l=0.53, tau_g=0.53, theta_eq=162.0, device="cuda:0")
+    s = AllenCahnSolver(cfg, dtype=getattr(torch, dtype), geo_amplification=1.5,
+                        geometric_wetting=True, storage=storage)
+    solid, frac = create_substrate_with_fraction(150, 150, 107, "ridge", 1.0, 22.5)
+    s.set_solid(solid, solid_fraction=frac)
+    phi0, _, u0 = create_fe_droplet_with_impact(150, 150, 107, center=(75, 75, 80),
+                                                radius=22.5, xi=4.0, u_impact=(0, 0, -0.05))
+    s.init_fields(phi0, u0)
+    return s, (solid, frac, phi0, u0)
+
+
+def cpu_baseline(inputs, nsteps=2):
+    """The numpy oracle (a single-threaded port of the reference step) on the
+    same inputs: `nsteps` steps of the full 150x150x107 case."""
+    from oracle.ac_oracle import OracleSolver, Params
+    solid, frac, phi0, u0 = inputs
+    p = Params(nx=150, ny=150, nz=107, rho_l=1.0, rho_g=1.0 / 828.0, sigma=0.01424, xi=4.0,
+               tau_l=0.53, tau_g=0.53, theta_eq=162.0)
+    o = OracleSolver(p, dtype=np.float32, geo_amplification=1.5, geometric_wetting=True)
+    o.set_solid(solid, frac)
+    o.init_fields(phi0, u0)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        o.step()
+    dt = time.perf_counter() - t0
+    n = 150 * 150 * 107
+    return {"value": round(n * nsteps / dt / 1e6, 4), "unit": "MLUPS", "cores": 1,
+            "kind": "port",
+            "sample": f"{nsteps} steps of the 150x150x107 ridge case, numpy oracle "
+                      f"(float32, single thread), {dt:.1f} s on "
+                      f"{platform.processor() or platform.machine()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--storage", default=None, choices=[None, "fp32", "bf16", "fp16"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the solver is a single-device code "
+              "(multi-GPU sharding is a design note, not built).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    print(f"[bench] building case on {torch.cuda.get_device_name(0)}", file=sys.stderr,
+          flush=True)
+    s, inputs = cfg2(args.storage)
+    n = 150 * 150 * 107
+    s.step(args.warmup)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    s.step(args.steps)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = start.elapsed_time(stop) / args.steps
+    mlups = n / (ms_step * 1e-3) / 1e6
+    print(f"[bench] {args.steps} steps: {ms_step * 1e3:.1f} us/step, {mlups:.0f} MLUPS "
+          f"(wall {wall:.2f} s)", file=sys.stderr, flush=True)
+    # per-kernel durations with events on the launch stream
+    prof_steps = min(100, max(10, args.steps // 5))
+    ta, tb, tstep = s.profile_steps(prof_steps)
+    achieved = B_COLLIDE * n / (tb * 1e-3) / 1e9
+    step_gbs = B_STEP * n / (ms_step * 1e-3) / 1e9
+    mass0, mass = s.mass()
+    meas = s.measure()
+    rec = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.storage in (None, "fp32") else f"f32-compute/{args.storage}-storage",
+        "data": "synthetic (deterministic droplet-impact initial condition, no RNG)",
+        "config": {"workload": "ridge impact, D3Q19 150x150x107, README construction, "
+                               "fp32, 2,407,500 nodes incl. solid",
+                   "grid": [150, 150, 107], "substrate": "ridge R*=1", "theta_eq": 162.0,
+                   "geo_amplification": 1.5, "boundary_relax": 0.3,
+                   "storage": args.storage or "fp32"},
+        "hbm_roofline_pct_step": round(100 * step_gbs / HBM_PEAK_GBS, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_collide (pass B)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_lu": B_COLLIDE, "avg_ms": round(tb, 5)},
+        "kernels_ms": {"k_phi": round(ta, 5), "k_collide": round(tb, 5),
+                       "step_events": round(tstep, 5)},
+        "mass_drift": abs(mass - mass0) / mass0,
+        "diag": {k: meas[k] for k in ("Dx", "Dy", "Dz", "k", "z_com", "nan")},
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        rec["cpu_baseline"] = cpu_baseline(inputs, args.cpu_steps)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

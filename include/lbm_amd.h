@@ -1,0 +1,180 @@
+/*
+ * lbm_amd.h -- C ABI of the MI355X Allen-Cahn phase-field LBM library.
+ *
+ * The reference (JunjunQing/pytorch-lbm) exposes this path as a Python class,
+ * not as an FFI.  Each entry point below replaces one method (or one
+ * observable attribute) of that class; the citation names the reference code
+ * it stands in for.  All pointers named *_dev are device pointers; "stream"
+ * is a hipStream_t passed as void* (NULL = the legacy default stream).  No
+ * entry point synchronises the stream unless its comment says so, so a
+ * sequence of lbm_step calls can be captured into a hipGraph.
+ *
+ * Precision: LBM_F32 and LBM_F64 compute and store in that type (the
+ * reference's dtype argument).  LBM_BF16 / LBM_F16 store both distribution
+ * sets as 16-bit deviations from the lattice-weight equilibrium and collide
+ * in fp32.  Mass reductions are always fp64.
+ *
+ * Return value: 0 on success, a negative LBM_E* code otherwise;
+ * lbm_last_error() gives a message for the calling thread.
+ */
+#ifndef LBM_AMD_H
+#define LBM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum lbm_precision { LBM_F32 = 0, LBM_F64 = 1, LBM_BF16 = 2, LBM_F16 = 3 };
+enum lbm_stab { LBM_STAB_FAKHARI = 0, LBM_STAB_LIANG = 1, LBM_STAB_NONE = 2 };
+enum lbm_frac_dtype { LBM_FRAC_NONE = 0, LBM_FRAC_F32 = 1, LBM_FRAC_F64 = 2 };
+
+enum lbm_status {
+  LBM_OK = 0,
+  LBM_EINVAL = -1,     /* bad argument / unsupported option */
+  LBM_ESTATE = -2,     /* call out of order (e.g. step before init) */
+  LBM_EHIP = -3,       /* HIP runtime error */
+  LBM_ENOMEM = -4      /* device allocation failed */
+};
+
+/* Observable fields for lbm_get_field.  Layouts are C-contiguous with the
+ * last grid axis fastest: scalar fields (nx,ny[,nz]); vectors (D,nx,ny[,nz]);
+ * distributions (Q,nx,ny[,nz]).  Element type is the run dtype (fp32 for the
+ * 16-bit storage forms), except the masks (uint8 0/1). */
+enum lbm_field {
+  LBM_FIELD_PHI = 0,        /* solver.phi   (mass-corrected, S12-relaxed)    */
+  LBM_FIELD_RHO = 1,        /* solver.rho   (from the start-of-step phi)     */
+  LBM_FIELD_U = 2,          /* solver.u                                      */
+  LBM_FIELD_P = 3,          /* solver.P     (sum of g at start of step)      */
+  LBM_FIELD_MU = 4,         /* solver.mu_phi                                 */
+  LBM_FIELD_F = 5,          /* solver.f     (post-stream, mass-corrected)    */
+  LBM_FIELD_G = 6,          /* solver.g                                      */
+  LBM_FIELD_PHI_RAW = 7,    /* start-of-step phi (clamped, 0 at solid)       */
+  LBM_FIELD_IS_BOUNDARY = 16, /* wetting.is_boundary (uint8)                 */
+  LBM_FIELD_NEAR_WALL = 17,   /* wetting.near_wall_mask (uint8)              */
+  LBM_FIELD_WALL_WEIGHT = 18, /* wetting.wall_weight (fp32 always)           */
+  LBM_FIELD_WALL_NORMAL = 19, /* wetting.wall_normal (fp32 always, D comps)  */
+  LBM_FIELD_PARTIAL = 20,     /* partial-solid mask (uint8)                  */
+  LBM_FIELD_SOLID = 21        /* solid mask (uint8)                          */
+};
+
+/* Constructor arguments: FEConfig (lbm/fe_config.py:27-150, after its
+ * __post_init__ derivations) plus the AllenCahnSolver keyword arguments
+ * (lbm/fe_ac_solver.py:54-58).  interface_mode / mu_drive / direct_wetting
+ * are not part of this ABI (only the default Allen-Cahn path is built). */
+typedef struct lbm_params {
+  int nx, ny, nz;            /* nz == 1 selects D2Q9, otherwise D3Q19 */
+  int precision;             /* enum lbm_precision */
+  double rho_l, rho_g, sigma, xi, beta, kappa;
+  double tau_l, tau_g, tau_h;
+  double theta_eq, phi_c;
+  double g_force[3];
+  int stab_mode;             /* enum lbm_stab */
+  double boundary_relax;     /* default 0.3 */
+  double sharpen_reduce;     /* default 0.8 */
+  double mu_wall_scale;      /* default 1.0 */
+  double ac_scale;           /* default 1.0 */
+  double ghost_scale;        /* <= 0: automatic (3 if |theta-90|>60 else 2) */
+  int geometric_wetting;     /* 0/1 */
+  double geo_amplification;  /* scalar alpha_geo (default 1.0) */
+} lbm_params;
+
+typedef struct lbm_solver lbm_solver;
+
+/* Fill *p with the reference's defaults for an nx*ny*nz grid. */
+void lbm_params_default(lbm_params* p, int nx, int ny, int nz);
+
+/* AllenCahnSolver.__init__ (lbm/fe_ac_solver.py:54-191): allocates the
+ * device state on the current HIP device. */
+int lbm_create(const lbm_params* p, lbm_solver** out);
+
+/* Releases everything lbm_create allocated. */
+int lbm_destroy(lbm_solver* s);
+
+/* AllenCahnSolver.set_solid (lbm/fe_ac_solver.py:193-237) including the
+ * BounceBack masks (lbm/boundary.py:15-34) and the ConningtonLeeWetting
+ * precompute (lbm/fe_wetting.py:122-311): boundary classification (periodic),
+ * wall normals (np.gradient of the fraction, or neighbour counting when
+ * frac_dev is NULL), near-wall distance/weight, per-direction wetting masks,
+ * partial-solid set.  solid_dev: uint8 0/1 per node.  Bit-exact with the
+ * reference for fp32 and fp64 fractions. */
+int lbm_set_geometry(lbm_solver* s, const uint8_t* solid_dev, const void* frac_dev,
+                     int frac_dtype, void* stream);
+
+/* Optional per-node alpha_geo field (tensor geo_amplification,
+ * lbm/fe_wetting.py:503-512); amp_dev in the run dtype (fp32 for 16-bit
+ * storage), NULL restores the scalar. */
+int lbm_set_amplification_field(lbm_solver* s, const void* amp_dev, void* stream);
+
+/* AllenCahnSolver.init_fields (lbm/fe_ac_solver.py:239-261).  phi_dev and
+ * u_dev (nullable: zero velocity) are in the run dtype (fp32 for 16-bit
+ * storage).  Computes M0 = sum(phi) in fp64 on the device. */
+int lbm_init_fields(lbm_solver* s, const void* phi_dev, const void* u_dev, void* stream);
+
+/* AllenCahnSolver.step() x nsteps (lbm/fe_ac_solver.py:380-639).  Two kernel
+ * launches per step, no host synchronisation. */
+int lbm_step(lbm_solver* s, int nsteps, void* stream);
+
+/* Capture `nsteps` steps (even, >= 2) into a HIP graph owned by the solver,
+ * replacing any earlier capture.  The solver must have stepped at least once
+ * since init_fields (so every replay starts from the same buffer parity and
+ * state form).  Nothing executes during capture.  The reference's own graph
+ * experiment is scripts/bench_cuda_graphs.py:58-91 (torch.cuda.graph around
+ * step()). */
+int lbm_graph_capture(lbm_solver* s, int nsteps);
+
+/* Launch the captured graph on `stream`: advances the solver by the captured
+ * number of steps. */
+int lbm_graph_launch(lbm_solver* s, void* stream);
+
+/* Measurement harness: runs `nsteps` steps on `stream` with HIP events
+ * around every kernel launch and writes the mean duration in milliseconds of
+ * pass A (out[0], phi + mass) and pass B (out[1], collide) and the mean step
+ * time (out[2]).  Synchronises `stream`.  No counterpart in the reference,
+ * which only times whole steps with time.time()
+ * (scripts/run_thesis_sweep.py:173,248). */
+int lbm_profile_steps(lbm_solver* s, int nsteps, void* stream, double* out);
+
+/* Observable attributes of the solver object (lbm/fe_ac_solver.py:97-105,
+ * lbm/fe_wetting.py:140,147,305-306): writes field `which` into dst_dev. */
+int lbm_get_field(lbm_solver* s, int which, void* dst_dev, void* stream);
+
+/* Size in bytes of lbm_get_field's output for `which` (0 if unknown). */
+size_t lbm_field_bytes(const lbm_solver* s, int which);
+
+/* AllenCahnSolver._lattice_gradient_and_laplacian (lbm/fe_ac_solver.py:
+ * 296-320) with the wetting ghost substitution (lbm/fe_wetting.py:418-446):
+ * phi_dev (N) -> grad_dev (D*N), lap_dev (N), run dtype. */
+int lbm_grad_lap(lbm_solver* s, const void* phi_dev, void* grad_dev, void* lap_dev,
+                 void* stream);
+
+/* Mass bookkeeping: M0 (phi_mass_init) and the fp64 sum of the last
+ * start-of-step phi.  Synchronises `stream`. */
+int lbm_get_mass(lbm_solver* s, double* m0, double* m_last, void* stream);
+
+/* Replace the mass target (inject_droplet resets it, lbm/fe_ac_solver.py:723). */
+int lbm_set_mass_target(lbm_solver* s, double m0);
+
+/* Number of completed steps (solver.step_count). */
+long long lbm_step_count(const lbm_solver* s);
+
+/* Diagnostics of lbm/fe_measure.py:11-82 and scripts/run_thesis_sweep.py:
+ * 185-206 on the current phi, reduced on the device:
+ * out[0..9] = Dx, Dy, Dz, k, volume, C_max, z_com, n_liquid, phi_sum, nan_flag.
+ * Synchronises `stream`. */
+int lbm_measure(lbm_solver* s, double* out, void* stream);
+
+/* Contact-line radius: max horizontal distance from (cx, cy) of nodes with
+ * phi > 0.5 in the first non-solid layer above the substrate at each column
+ * (3D only).  Synchronises `stream`. */
+int lbm_contact_radius(lbm_solver* s, double cx, double cy, double* out, void* stream);
+
+/* Thread-local message for the last failed call. */
+const char* lbm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_AMD_H */

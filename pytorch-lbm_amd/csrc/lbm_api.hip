@@ -1,0 +1,733 @@
+// Host side of the C ABI declared in include/lbm_amd.h.
+//
+// One lbm_solver owns its device state (allocated once in lbm_create):
+//   Fb[2], Gb[2]  distribution sets, ping-pong (post-collision after a step)
+//   u             velocity (D components), updated in place by pass B
+//   phi[2]        start-of-step phi, by step parity
+//   meta, wmask, normal, eps, dist, surf   geometry (set_geometry)
+//   ctrl          M0, per-parity mass and mass factor, reduction counters
+// lbm_step enqueues two kernels per step on the caller's stream and never
+// synchronises, so step sequences are graph-capturable.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/lbm_amd.h"
+#include "lbm_kernels.hpp"
+#include "lbm_geometry.hpp"
+#include "lbm_measure.hpp"
+
+using namespace lbm;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(LBM_EHIP, std::string(#expr " failed: ") + hipGetErrorString(_e));   \
+  } while (0)
+
+constexpr int BS = 256;
+
+struct lbm_solver {
+  lbm_params p;
+  int dim = 3, Q = 19;
+  Grid g{};
+  long long N = 0;
+  size_t esz = 4;            // bytes per stored/computed element
+  int wet = WET_NONE;
+  double hk = 0.0, mu_wall = 0.0, pck = 0.0, target = 0.0;
+  bool has_target = false;
+  bool corr_on = false;
+  bool near_wall_any = false;
+  void* Fb[2] = {nullptr, nullptr};
+  void* Gb[2] = {nullptr, nullptr};
+  void* u = nullptr;
+  void* phi[2] = {nullptr, nullptr};
+  void* eps = nullptr;
+  void* ampf = nullptr;
+  uint8_t* meta = nullptr;
+  uint8_t* bnd = nullptr;
+  uint8_t* dist[2] = {nullptr, nullptr};
+  uint8_t* solid = nullptr;
+  uint32_t* wmask = nullptr;
+  float* normal = nullptr;
+  int16_t* surf = nullptr;
+  double* partials = nullptr;
+  void* mparts = nullptr;
+  double* dout = nullptr;     // device scratch for measure outputs
+  Ctrl* ctrl = nullptr;
+  unsigned int nblocks = 0;
+  // host-side state machine
+  bool geom = false, inited = false;
+  long long k = 0;            // completed steps
+  int cur = 0;                // buffer index holding the latest distributions
+  bool local = true;          // buffers at cur hold post-stream values
+  bool prev_local = true;     // form of buffers at cur^1 (source of P)
+  bool phiu_valid = false;
+  bool has_frac = false;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int gsteps = 0;
+};
+
+// ---------------------------------------------------------------------------
+static void fast_div(uint32_t d, uint32_t* m, uint32_t* sh) {
+  if (d <= 1) { *m = 0; *sh = 0; return; }
+  int cl = 0;
+  while ((1u << cl) < d) ++cl;
+  const int p = 31 + cl;
+  *m = (uint32_t)(((1ull << p) + d - 1) / d);
+  *sh = (uint32_t)(p - 32);
+}
+
+static Grid make_grid(int nx, int ny, int nz) {
+  Grid g{};
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  if (nz > 1) { g.sx = ny * nz; g.sy = nz; }
+  else { g.sx = ny; g.sy = 1; }
+  g.N = (long long)nx * ny * nz;
+  fast_div((uint32_t)nz, &g.mz, &g.sz_shift);
+  fast_div((uint32_t)ny, &g.my, &g.sy_shift);
+  return g;
+}
+
+template <typename T> static T TT(double v) { return (T)v; }
+
+template <int DIM> static float w32(int i) { return W32<DIM>(i); }
+
+// Physics constants in the run precision (see lbm_phys.hpp).
+template <typename T, int DIM> static Phys<T> make_phys(const lbm_solver* s) {
+  const lbm_params& p = s->p;
+  constexpr int Q = Lat<DIM>::Q;
+  Phys<T> ph;
+  std::memset(&ph, 0, sizeof(ph));
+  const double cs2 = 1.0 / 3.0;
+  ph.cs2 = TT<T>(cs2);
+  ph.two_cs2 = TT<T>(2.0 * cs2);
+  for (int i = 0; i < Q; ++i) {
+    ph.w[i] = (T)w32<DIM>(i);
+    const T woc = ph.w[i] / ph.cs2;                 // self._w / self.cs2
+    for (int d = 0; d < DIM; ++d) ph.gcoef[i][d] = woc * (T)E<DIM>(i, d);
+    ph.lcoef[i] = T(2) * woc;                        // 2.0 * wi
+    ph.linit[i] = T(2) * ph.w[i];                    // 2.0 * self._w[i]
+  }
+  ph.four_beta = TT<T>(4.0 * p.beta);
+  ph.kappa = TT<T>(p.kappa);
+  ph.rho_l = TT<T>(p.rho_l);
+  ph.rho_g = TT<T>(p.rho_g);
+  ph.rho_floor = TT<T>(0.5 * p.rho_l);
+  ph.drho = TT<T>(p.rho_l - p.rho_g);
+  ph.visc = TT<T>((0.5 - p.tau_g) / p.tau_g);
+  ph.xi = TT<T>(p.xi);
+  ph.ac_scale = TT<T>(p.ac_scale);
+  ph.wet = s->wet;
+  ph.mu_wall_add = TT<T>(s->mu_wall * p.mu_wall_scale);
+  ph.phi_c = TT<T>(p.phi_c);
+  ph.hk32 = (float)s->hk;
+  ph.pck32 = (float)s->pck;
+  const double th = p.theta_eq * M_PI / 180.0;
+  ph.cos_t = TT<T>(std::cos(th));
+  ph.sin_t = TT<T>(std::sin(th));
+  ph.amp = TT<T>(p.geo_amplification);
+  ph.amp_on = p.geo_amplification > 1.0 ? 1 : 0;
+  ph.corr_on = s->corr_on ? 1 : 0;
+  // sharpening factor per distance class: 1 - reduce * T(wall_weight)
+  for (int c = 0; c < 16; ++c) {
+    if (s->wet == WET_NONE) { ph.sfac[c] = T(1); continue; }
+    const float arg = -(float)(c <= 11 ? c : 100) / 2.5f;
+    const float ww = (float)std::exp((double)arg);   // correctly rounded fp32 exp
+    ph.sfac[c] = T(1) - TT<T>(p.sharpen_reduce) * (T)ww;
+  }
+  const double tau_f = p.tau_h > 0 ? p.tau_h : p.tau_l;
+  ph.acoef = TT<T>(1.0 - 0.5 / tau_f);
+  ph.om_f = TT<T>(1.0 / tau_f);
+  const T tl = TT<T>(p.tau_l);
+  ph.ns_c = T(1) - T(0.5) / tl;
+  ph.omg_c = T(1) / tl;
+  ph.var_tau = p.tau_g != p.tau_l ? 1 : 0;
+  ph.tl_m = TT<T>(p.tau_l - 0.5);
+  ph.tg_m = TT<T>(p.tau_g - 0.5);
+  bool body = false;
+  for (int d = 0; d < DIM; ++d) {
+    ph.gforce[d] = TT<T>(p.g_force[d]);
+    if (p.g_force[d] != 0.0) body = true;
+  }
+  ph.body_on = body ? 1 : 0;
+  ph.stab = p.stab_mode;
+  ph.s12_on = (s->wet != WET_NONE && p.boundary_relax > 0 && s->has_target &&
+               std::fabs(s->target - 0.5) > 0.05) ? 1 : 0;
+  ph.s12_r = TT<T>(p.boundary_relax);
+  ph.s12_t = TT<T>(s->target);
+  ph.thr_lo = TT<T>(0.15);
+  ph.thr_hi = TT<T>(0.85);
+  return ph;
+}
+
+static unsigned int nblk(long long n) { return (unsigned int)((n + BS - 1) / BS); }
+
+// ---------------------------------------------------------------------------
+// Exported entry points: C linkage comes from the declarations in lbm_amd.h.
+
+const char* lbm_last_error(void) { return g_err.c_str(); }
+
+void lbm_params_default(lbm_params* p, int nx, int ny, int nz) {
+  std::memset(p, 0, sizeof(*p));
+  p->nx = nx; p->ny = ny; p->nz = nz;
+  p->precision = LBM_F64;
+  p->rho_l = 828.0; p->rho_g = 1.0; p->sigma = 0.01; p->xi = 5.0;
+  p->tau_l = 0.0; p->tau_g = 0.6; p->tau_h = 1.0;
+  p->theta_eq = 162.0;
+  p->stab_mode = LBM_STAB_FAKHARI;
+  p->boundary_relax = 0.3; p->sharpen_reduce = 0.8; p->mu_wall_scale = 1.0;
+  p->ac_scale = 1.0; p->ghost_scale = 0.0;
+  p->geometric_wetting = 0; p->geo_amplification = 1.0;
+}
+
+int lbm_create(const lbm_params* pin, lbm_solver** out) {
+  if (!pin || !out) return fail(LBM_EINVAL, "null argument");
+  const lbm_params& p = *pin;
+  if (p.nx < 2 || p.ny < 2 || p.nz < 1)
+    return fail(LBM_EINVAL, "grid must be at least 2 x 2 (x 2 in 3D)");
+  if (p.nz == 2) return fail(LBM_EINVAL, "nz must be 1 (2D) or >= 3");
+  if (p.precision != LBM_F32 && p.precision != LBM_F64)
+    return fail(LBM_EINVAL, "precision not supported by this entry point");
+  if (!(p.tau_l > 0.5) || !(p.tau_g > 0.5))
+    return fail(LBM_EINVAL, "tau_l and tau_g must exceed 0.5");
+  if (p.stab_mode < 0 || p.stab_mode > 2) return fail(LBM_EINVAL, "bad stab_mode");
+  const long long N = (long long)p.nx * p.ny * p.nz;
+  if (N >= (1ll << 31) / 19) return fail(LBM_EINVAL, "grid too large for 32-bit indexing");
+  lbm_solver* s = new lbm_solver();
+  s->p = p;
+  s->dim = p.nz == 1 ? 2 : 3;
+  s->Q = s->dim == 3 ? 19 : 9;
+  s->g = make_grid(p.nx, p.ny, p.nz);
+  s->N = N;
+  s->esz = p.precision == LBM_F64 ? 8 : 4;
+  // wetting mode (lbm/fe_ac_solver.py:223-236, lbm/fe_wetting.py:85-115)
+  if (p.theta_eq != 90.0 && p.kappa > 0) {
+    s->wet = (p.theta_eq > 130.0 || p.theta_eq < 50.0) ? WET_LINEAR : WET_QUADRATIC;
+    if (s->wet == WET_LINEAR) {
+      const double h = p.sigma * std::cos(p.theta_eq * M_PI / 180.0);
+      double gs = p.ghost_scale;
+      if (!(gs > 0)) gs = std::fabs(p.theta_eq - 90.0) > 60.0 ? 3.0 : 2.0;
+      s->hk = h / p.kappa * gs;
+      s->mu_wall = -h * 2.0;
+    } else {
+      s->pck = p.phi_c / p.kappa;
+    }
+  }
+  // boundary-relax target (lbm/fe_ac_solver.py:165-177)
+  if (p.theta_eq != 90.0) {
+    const double c = std::cos(p.theta_eq * M_PI / 180.0);
+    const double disc = 1.0 - 4.0 * (-c / 3.0);
+    s->target = disc >= 0 ? 0.5 * (1.0 - std::sqrt(disc))
+                          : 0.5 * (1.0 - std::cos(M_PI - p.theta_eq * M_PI / 180.0));
+    s->has_target = true;
+  }
+  const size_t es = s->esz, Q = s->Q, D = s->dim;
+  s->nblocks = nblk(N);
+  hipError_t e = hipSuccess;
+#define ALLOC(ptr, bytes)                                         \
+  if (e == hipSuccess) {                                          \
+    e = hipMalloc((void**)&(ptr), (bytes));                       \
+    if (e == hipSuccess) e = hipMemset((ptr), 0, (bytes));        \
+  }
+  for (int b = 0; b < 2; ++b) {
+    ALLOC(s->Fb[b], Q * N * es);
+    ALLOC(s->Gb[b], Q * N * es);
+    ALLOC(s->phi[b], N * es);
+    ALLOC(s->dist[b], N);
+  }
+  ALLOC(s->u, D * N * es);
+  ALLOC(s->eps, N * es);
+  ALLOC(s->meta, N);
+  ALLOC(s->bnd, N);
+  ALLOC(s->solid, N);
+  ALLOC(s->wmask, N * 4);
+  ALLOC(s->normal, D * N * 4);
+  ALLOC(s->surf, (size_t)p.nx * p.ny * 2);
+  ALLOC(s->partials, (size_t)s->nblocks * 8);
+  ALLOC(s->mparts, (size_t)s->nblocks * sizeof(MeasRec));
+  ALLOC(s->dout, 16 * 8);
+  ALLOC(s->ctrl, sizeof(Ctrl));
+#undef ALLOC
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    lbm_destroy(s);
+    return fail(e == hipErrorOutOfMemory ? LBM_ENOMEM : LBM_EHIP,
+                std::string("allocation failed: ") + hipGetErrorString(e));
+  }
+  *out = s;
+  return LBM_OK;
+}
+
+int lbm_destroy(lbm_solver* s) {
+  if (!s) return LBM_OK;
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  void* ptrs[] = {s->Fb[0], s->Fb[1], s->Gb[0], s->Gb[1], s->u, s->phi[0], s->phi[1],
+                  s->eps, s->ampf, s->meta, s->bnd, s->dist[0], s->dist[1], s->solid,
+                  s->wmask, s->normal, s->surf, s->partials, s->mparts, s->dout, s->ctrl};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete s;
+  return LBM_OK;
+}
+
+template <typename T, int DIM>
+static int set_geometry_impl(lbm_solver* s, const void* frac, int fdt, hipStream_t st) {
+  const Grid g = s->g;
+  const unsigned nb = s->nblocks;
+  k_boundary<DIM, BS><<<nb, BS, 0, st>>>(g, s->solid, s->bnd, s->wmask);
+  if (fdt == LBM_FRAC_F64)
+    k_normals_frac<double, DIM, BS><<<nb, BS, 0, st>>>(g, (const double*)frac, s->bnd, s->normal);
+  else if (fdt == LBM_FRAC_F32)
+    k_normals_frac<float, DIM, BS><<<nb, BS, 0, st>>>(g, (const float*)frac, s->bnd, s->normal);
+  else
+    k_normals_discrete<DIM, BS><<<nb, BS, 0, st>>>(g, s->solid, s->bnd, s->normal);
+  k_dist_seed<BS><<<nb, BS, 0, st>>>(s->N, s->bnd, s->dist[0]);
+  int a = 0;
+  for (int pass = 0; pass < 11; ++pass) {   // int(2 * xi_wall) + 1, xi_wall = 5
+    k_dist_pass<DIM, BS><<<nb, BS, 0, st>>>(g, s->dist[a], s->dist[a ^ 1]);
+    a ^= 1;
+  }
+  if (a != 0)
+    HIPCHK(hipMemcpyAsync(s->dist[0], s->dist[1], s->N, hipMemcpyDeviceToDevice, st));
+  const int wet = s->wet != WET_NONE;
+  if (fdt == LBM_FRAC_F64)
+    k_meta<T, double, DIM, BS><<<nb, BS, 0, st>>>(g, s->solid, s->bnd, s->dist[0],
+                                                  (const double*)frac, s->normal, s->meta,
+                                                  (T*)s->eps, wet);
+  else
+    k_meta<T, float, DIM, BS><<<nb, BS, 0, st>>>(g, s->solid, s->bnd, s->dist[0],
+                                                 (const float*)frac, s->normal, s->meta,
+                                                 (T*)s->eps, wet);
+  const long long ncol = DIM == 3 ? (long long)g.nx * g.ny : g.nx;
+  k_surface<DIM, BS><<<nblk(ncol), BS, 0, st>>>(g, s->solid, s->surf);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+#define DISPATCH(s, CALL)                                                  \
+  (((s)->esz == 8) ? (((s)->dim == 3) ? CALL(double, 3) : CALL(double, 2)) \
+                   : (((s)->dim == 3) ? CALL(float, 3) : CALL(float, 2)))
+
+int lbm_set_geometry(lbm_solver* s, const uint8_t* solid_dev, const void* frac_dev,
+                     int frac_dtype, void* stream) {
+  if (!s || !solid_dev) return fail(LBM_EINVAL, "null argument");
+  if (frac_dtype != LBM_FRAC_NONE && !frac_dev) return fail(LBM_EINVAL, "null fraction");
+  if (frac_dtype == LBM_FRAC_NONE) frac_dev = nullptr;
+  if (frac_dtype < 0 || frac_dtype > 2) return fail(LBM_EINVAL, "bad fraction dtype");
+  hipStream_t st = (hipStream_t)stream;
+  if (solid_dev != s->solid)
+    HIPCHK(hipMemcpyAsync(s->solid, solid_dev, s->N, hipMemcpyDeviceToDevice, st));
+  s->has_frac = frac_dev != nullptr;
+#define CALL(T, D) set_geometry_impl<T, D>(s, frac_dev, frac_dev ? frac_dtype : 0, st)
+  int rc = DISPATCH(s, CALL);
+#undef CALL
+  if (rc) return rc;
+  // near_wall_any: any non-solid node within distance < 10 exists iff there is
+  // at least one boundary node; read back one flag (one-time, synchronous).
+  int any = 0;
+  if (s->wet != WET_NONE) {
+    uint8_t* tmp = nullptr;
+    HIPCHK(hipHostMalloc((void**)&tmp, s->N, hipHostMallocDefault));
+    hipError_t e = hipMemcpyAsync(tmp, s->bnd, s->N, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess)
+      for (long long i = 0; i < s->N; ++i)
+        if (tmp[i]) { any = 1; break; }
+    (void)hipHostFree(tmp);
+    HIPCHK(e);
+  }
+  s->near_wall_any = any != 0;
+  const double th = s->p.theta_eq * M_PI / 180.0;
+  s->corr_on = s->wet != WET_NONE && s->p.geometric_wetting && s->near_wall_any &&
+               std::fabs(std::sin(th)) >= 1e-10;
+  s->geom = true;
+  return LBM_OK;
+}
+
+int lbm_set_amplification_field(lbm_solver* s, const void* amp_dev, void* stream) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  hipStream_t st = (hipStream_t)stream;
+  if (!amp_dev) {
+    if (s->ampf) {
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipFree(s->ampf));
+    }
+    s->ampf = nullptr;
+    return LBM_OK;
+  }
+  if (!s->ampf) HIPCHK(hipMalloc(&s->ampf, s->N * s->esz));
+  HIPCHK(hipMemcpyAsync(s->ampf, amp_dev, s->N * s->esz, hipMemcpyDeviceToDevice, st));
+  return LBM_OK;
+}
+
+template <typename T, int DIM>
+static int init_impl(lbm_solver* s, const void* phi, const void* u, hipStream_t st) {
+  const Phys<T> ph = make_phys<T, DIM>(s);
+  k_init<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(s->g, ph, (const T*)phi, (const T*)u,
+                                                (T*)s->phi[0], (T*)s->u, (T*)s->Fb[0],
+                                                (T*)s->Gb[0], s->partials, s->ctrl);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_init_fields(lbm_solver* s, const void* phi_dev, const void* u_dev, void* stream) {
+  if (!s || !phi_dev) return fail(LBM_EINVAL, "null argument");
+  if (!s->geom) {
+    // the reference allows init before set_solid: an all-fluid domain
+    HIPCHK(hipMemsetAsync(s->solid, 0, s->N, (hipStream_t)stream));
+    int rc = lbm_set_geometry(s, s->solid, nullptr, LBM_FRAC_NONE, stream);
+    if (rc) return rc;
+  }
+#define CALL(T, D) init_impl<T, D>(s, phi_dev, u_dev, (hipStream_t)stream)
+  int rc = DISPATCH(s, CALL);
+#undef CALL
+  if (rc) return rc;
+  s->inited = true;
+  s->k = 0;
+  s->cur = 0;
+  s->local = true;
+  s->prev_local = true;
+  s->phiu_valid = false;
+  return LBM_OK;
+}
+
+template <typename T, int DIM>
+static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev = nullptr) {
+  const Phys<T> ph = make_phys<T, DIM>(s);
+  for (int it = 0; it < nsteps; ++it) {
+    if (ev) HIPCHK(hipEventRecord(ev[3 * it + 0], st));
+    const int src = s->cur, dst = s->cur ^ 1;
+    const int pslot = (int)(s->k & 1), nslot = (int)((s->k + 1) & 1);
+    k_phi<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(
+        s->g, ph, (const T*)s->Fb[src], (const T*)s->phi[pslot], (T*)s->phi[nslot], s->meta,
+        (const T*)s->eps, s->partials, s->ctrl, pslot, nslot, s->local ? 1 : 0);
+    if (ev) HIPCHK(hipEventRecord(ev[3 * it + 1], st));
+    CollideArgs<T, DIM> a;
+    a.g = s->g;
+    a.Fin = (const T*)s->Fb[src];
+    a.Gin = (const T*)s->Gb[src];
+    a.Fout = (T*)s->Fb[dst];
+    a.Gout = (T*)s->Gb[dst];
+    a.u = (T*)s->u;
+    a.phi_cur = (const T*)s->phi[nslot];
+    a.phi_prev = (const T*)s->phi[pslot];
+    a.meta = s->meta;
+    a.wmask = s->wmask;
+    a.normal = s->normal;
+    a.eps = (const T*)s->eps;
+    a.ampf = (const T*)s->ampf;
+    a.ctrl = s->ctrl;
+    a.slot_prev = pslot;
+    a.local = s->local ? 1 : 0;
+    a.phiu_valid = s->phiu_valid ? 1 : 0;
+    k_collide<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(a, ph);
+    if (ev) HIPCHK(hipEventRecord(ev[3 * it + 2], st));
+    s->prev_local = s->local;
+    s->local = false;
+    s->cur = dst;
+    s->k += 1;
+    s->phiu_valid = true;
+  }
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_step(lbm_solver* s, int nsteps, void* stream) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  if (!s->inited) return fail(LBM_ESTATE, "step() before init_fields()");
+  if (nsteps < 0) return fail(LBM_EINVAL, "negative step count");
+#define CALL(T, D) step_impl<T, D>(s, nsteps, (hipStream_t)stream)
+  return DISPATCH(s, CALL);
+#undef CALL
+}
+
+int lbm_graph_capture(lbm_solver* s, int nsteps) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  if (!s->inited || s->k < 1 || s->local)
+    return fail(LBM_ESTATE, "graph capture needs at least one eager step after init");
+  if (nsteps < 2 || (nsteps & 1)) return fail(LBM_EINVAL, "nsteps must be even and >= 2");
+  if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+  if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+  hipStream_t cs;
+  HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  // snapshot the host state machine: capture records launches only
+  const long long k0 = s->k;
+  const int cur0 = s->cur;
+  const bool l0 = s->local, pl0 = s->prev_local, pv0 = s->phiu_valid;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  int rc = LBM_OK;
+  if (e == hipSuccess) {
+    rc = lbm_step(s, nsteps, cs);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    if (rc == LBM_OK && e2 == hipSuccess) {
+      s->graph = g;
+      e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+    } else {
+      if (g) (void)hipGraphDestroy(g);
+      if (e2 != hipSuccess) e = e2;
+    }
+  }
+  s->k = k0; s->cur = cur0; s->local = l0; s->prev_local = pl0; s->phiu_valid = pv0;
+  (void)hipStreamDestroy(cs);
+  if (rc != LBM_OK) return rc;
+  if (e != hipSuccess) return fail(LBM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+  s->gsteps = nsteps;
+  return LBM_OK;
+}
+
+int lbm_graph_launch(lbm_solver* s, void* stream) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  if (!s->gexec) return fail(LBM_ESTATE, "no captured graph");
+  if (s->local || !s->phiu_valid) return fail(LBM_ESTATE, "state changed since capture");
+  HIPCHK(hipGraphLaunch(s->gexec, (hipStream_t)stream));
+  s->k += s->gsteps;        // even: buffer parity unchanged
+  s->prev_local = false;
+  return LBM_OK;
+}
+
+int lbm_profile_steps(lbm_solver* s, int nsteps, void* stream, double* out) {
+  if (!s || !out) return fail(LBM_EINVAL, "null argument");
+  if (!s->inited) return fail(LBM_ESTATE, "step() before init_fields()");
+  if (nsteps < 1) return fail(LBM_EINVAL, "nsteps must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<hipEvent_t> ev((size_t)nsteps * 3, nullptr);
+  int rc = LBM_OK;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) rc = fail(LBM_EHIP, "hipEventCreate failed");
+  if (rc == LBM_OK) {
+#define CALL(T, D) step_impl<T, D>(s, nsteps, st, ev.data())
+    rc = DISPATCH(s, CALL);
+#undef CALL
+  }
+  if (rc == LBM_OK && hipStreamSynchronize(st) != hipSuccess)
+    rc = fail(LBM_EHIP, "stream synchronize failed");
+  double ta = 0, tb = 0, tstep = 0;
+  if (rc == LBM_OK) {
+    for (int i = 0; i < nsteps; ++i) {
+      float a = 0, b = 0;
+      (void)hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]);
+      (void)hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]);
+      ta += a; tb += b;
+      if (i > 0) {
+        float c = 0;
+        (void)hipEventElapsedTime(&c, ev[3 * (i - 1)], ev[3 * i]);
+        tstep += c;
+      }
+    }
+    out[0] = ta / nsteps;
+    out[1] = tb / nsteps;
+    out[2] = nsteps > 1 ? tstep / (nsteps - 1) : ta + tb;
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return rc;
+}
+
+size_t lbm_field_bytes(const lbm_solver* s, int which) {
+  if (!s) return 0;
+  const size_t N = (size_t)s->N, es = s->esz, Q = s->Q, D = s->dim;
+  switch (which) {
+    case LBM_FIELD_PHI: case LBM_FIELD_RHO: case LBM_FIELD_P: case LBM_FIELD_MU:
+    case LBM_FIELD_PHI_RAW:
+      return N * es;
+    case LBM_FIELD_U: return D * N * es;
+    case LBM_FIELD_F: case LBM_FIELD_G: return Q * N * es;
+    case LBM_FIELD_IS_BOUNDARY: case LBM_FIELD_NEAR_WALL: case LBM_FIELD_PARTIAL:
+    case LBM_FIELD_SOLID:
+      return N;
+    case LBM_FIELD_WALL_WEIGHT: return N * 4;
+    case LBM_FIELD_WALL_NORMAL: return D * N * 4;
+  }
+  return 0;
+}
+
+// geometry observables from the packed byte
+template <int BS_>
+__global__ void k_geom_obs(long long N, const uint8_t* __restrict__ meta,
+                           const uint8_t* __restrict__ bnd, int which, void* out) {
+  const long long x = (long long)blockIdx.x * BS_ + threadIdx.x;
+  if (x >= N) return;
+  const uint8_t m = meta[x];
+  const bool solid = m & M_SOLID;
+  const int cls = dist_class(m);
+  switch (which) {
+    case LBM_FIELD_IS_BOUNDARY: ((uint8_t*)out)[x] = bnd[x]; break;
+    case LBM_FIELD_NEAR_WALL: ((uint8_t*)out)[x] = (!solid && cls < 10) ? 1 : 0; break;
+    case LBM_FIELD_PARTIAL: ((uint8_t*)out)[x] = (m & M_PARTIAL) ? 1 : 0; break;
+    case LBM_FIELD_SOLID: ((uint8_t*)out)[x] = solid ? 1 : 0; break;
+    case LBM_FIELD_WALL_WEIGHT: {
+      float w = 0.0f;
+      if (!solid) {
+        const float arg = -(float)(cls <= 11 ? cls : 100) / 2.5f;
+        w = (float)exp((double)arg);
+      }
+      ((float*)out)[x] = w;
+      break;
+    }
+  }
+}
+
+template <typename T, int DIM>
+static int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st) {
+  const Phys<T> ph = make_phys<T, DIM>(s);
+  ObsArgs<T, DIM> a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = s->g;
+  a.phi_cur = (const T*)s->phi[s->k & 1];
+  a.phi_prev = (const T*)s->phi[(s->k + 1) & 1];
+  a.meta = s->meta;
+  a.wmask = s->wmask;
+  a.normal = s->normal;
+  a.eps = (const T*)s->eps;
+  a.ctrl = s->ctrl;
+  a.out = (T*)dst;
+  a.slot = (int)(s->k & 1);
+  a.scaled = s->k > 0 ? 1 : 0;
+  a.local = s->local ? 1 : 0;
+  const size_t N = (size_t)s->N;
+  switch (which) {
+    case LBM_FIELD_PHI: a.kind = OBS_PHI; break;
+    case LBM_FIELD_PHI_RAW:
+      HIPCHK(hipMemcpyAsync(dst, a.phi_cur, N * s->esz, hipMemcpyDeviceToDevice, st));
+      return LBM_OK;
+    case LBM_FIELD_RHO: a.kind = OBS_RHO; break;
+    case LBM_FIELD_U:
+      HIPCHK(hipMemcpyAsync(dst, s->u, (size_t)s->dim * N * s->esz, hipMemcpyDeviceToDevice, st));
+      return LBM_OK;
+    case LBM_FIELD_MU:
+      a.kind = OBS_MU;
+      a.init_lap = s->k == 0 ? 1 : 0;
+      break;
+    case LBM_FIELD_F: a.kind = OBS_F; a.Fsrc = (const T*)s->Fb[s->cur]; break;
+    case LBM_FIELD_G: a.kind = OBS_G; a.Fsrc = (const T*)s->Gb[s->cur]; break;
+    case LBM_FIELD_P:
+      if (s->k == 0) {
+        HIPCHK(hipMemsetAsync(dst, 0, N * s->esz, st));
+        return LBM_OK;
+      }
+      a.kind = OBS_P;
+      a.Fsrc = (const T*)s->Gb[s->cur ^ 1];
+      a.local = s->prev_local ? 1 : 0;
+      break;
+    case LBM_FIELD_WALL_NORMAL:
+      HIPCHK(hipMemcpyAsync(dst, s->normal, (size_t)s->dim * N * 4, hipMemcpyDeviceToDevice, st));
+      return LBM_OK;
+    case LBM_FIELD_IS_BOUNDARY: case LBM_FIELD_NEAR_WALL: case LBM_FIELD_WALL_WEIGHT:
+    case LBM_FIELD_PARTIAL: case LBM_FIELD_SOLID:
+      k_geom_obs<BS><<<s->nblocks, BS, 0, st>>>(s->N, s->meta, s->bnd, which, dst);
+      HIPCHK(hipGetLastError());
+      return LBM_OK;
+    default:
+      return fail(LBM_EINVAL, "unknown field");
+  }
+  k_observe<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(a, ph);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_get_field(lbm_solver* s, int which, void* dst_dev, void* stream) {
+  if (!s || !dst_dev) return fail(LBM_EINVAL, "null argument");
+  if (!s->geom) return fail(LBM_ESTATE, "geometry not set");
+  const bool geo = which >= LBM_FIELD_IS_BOUNDARY;
+  if (!geo && !s->inited) return fail(LBM_ESTATE, "fields not initialised");
+#define CALL(T, D) get_field_impl<T, D>(s, which, dst_dev, (hipStream_t)stream)
+  return DISPATCH(s, CALL);
+#undef CALL
+}
+
+template <typename T, int DIM>
+static int gradlap_impl(lbm_solver* s, const void* phi, void* grad, void* lap, hipStream_t st) {
+  const Phys<T> ph = make_phys<T, DIM>(s);
+  k_gradlap<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(s->g, ph, (const T*)phi,
+                                                   s->geom ? s->meta : nullptr, s->wmask,
+                                                   s->normal, (T*)grad, (T*)lap);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_grad_lap(lbm_solver* s, const void* phi_dev, void* grad_dev, void* lap_dev,
+                 void* stream) {
+  if (!s || !phi_dev || !grad_dev || !lap_dev) return fail(LBM_EINVAL, "null argument");
+#define CALL(T, D) gradlap_impl<T, D>(s, phi_dev, grad_dev, lap_dev, (hipStream_t)stream)
+  return DISPATCH(s, CALL);
+#undef CALL
+}
+
+int lbm_get_mass(lbm_solver* s, double* m0, double* m_last, void* stream) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  Ctrl c;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (m0) *m0 = c.M0;
+  if (m_last) *m_last = s->k > 0 ? c.mass[s->k & 1] : c.M0;
+  return LBM_OK;
+}
+
+int lbm_set_mass_target(lbm_solver* s, double m0) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  HIPCHK(hipMemcpy(&s->ctrl->M0, &m0, sizeof(double), hipMemcpyHostToDevice));
+  return LBM_OK;
+}
+
+long long lbm_step_count(const lbm_solver* s) { return s ? s->k : -1; }
+
+
+template <typename T, int DIM>
+static int measure_impl(lbm_solver* s, double* out, hipStream_t st) {
+  const Phys<T> ph = make_phys<T, DIM>(s);
+  k_measure<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(
+      s->g, ph, (const T*)s->phi[s->k & 1], s->meta, s->ctrl, (int)(s->k & 1),
+      s->k > 0 ? 1 : 0, (MeasRec*)s->mparts, &s->ctrl->counter2, s->dout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->dout, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_measure(lbm_solver* s, double* out, void* stream) {
+  if (!s || !out) return fail(LBM_EINVAL, "null argument");
+  if (!s->inited) return fail(LBM_ESTATE, "fields not initialised");
+#define CALL(T, D) measure_impl<T, D>(s, out, (hipStream_t)stream)
+  return DISPATCH(s, CALL);
+#undef CALL
+}
+
+template <typename T, int DIM>
+static int contact_impl(lbm_solver* s, double cx, double cy, double* out, hipStream_t st) {
+  const Phys<T> ph = make_phys<T, DIM>(s);
+  const long long ncol = (long long)s->g.nx * s->g.ny;
+  k_contact<T, DIM, BS><<<nblk(ncol), BS, 0, st>>>(
+      s->g, ph, (const T*)s->phi[s->k & 1], s->meta, s->surf, s->ctrl, (int)(s->k & 1),
+      s->k > 0 ? 1 : 0, cx, cy, s->partials, &s->ctrl->counter2, s->dout + 12);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->dout + 12, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_contact_radius(lbm_solver* s, double cx, double cy, double* out, void* stream) {
+  if (!s || !out) return fail(LBM_EINVAL, "null argument");
+  if (!s->inited) return fail(LBM_ESTATE, "fields not initialised");
+  if (s->dim != 3) return fail(LBM_EINVAL, "contact radius is defined for 3D grids");
+#define CALL(T, D) contact_impl<T, D>(s, cx, cy, out, (hipStream_t)stream)
+  return DISPATCH(s, CALL);
+#undef CALL
+}

@@ -1,0 +1,210 @@
+// One-time geometry precompute on the device: the classification behind
+// set_solid (lbm/fe_ac_solver.py:193-237, lbm/boundary.py:15-34) and the
+// ConningtonLeeWetting constructor (lbm/fe_wetting.py:122-311).
+//
+// Every quantity is reproduced bit-exactly:
+//  * boundary(x) = !solid(x) && solid(x + e_i) for some i >= 1 (periodic);
+//  * wetting mask bit i = boundary(x) && solid(x + e_i);
+//  * smooth normals: numpy.gradient of the fraction (central differences
+//    inside, one-sided at the array ends, not periodic) in the fraction's own
+//    precision, n = (-grad/|grad|) * boundary rounded to fp32;
+//  * discrete normals (no fraction): minus the mean of the solid-neighbour
+//    directions, in-bounds neighbours only, normalised in fp64, rounded to fp32;
+//  * near-wall distance: 11 min-plus passes over the face neighbours
+//    (periodic), integer valued, 100 at solid and unreached nodes.
+#pragma once
+
+#include "lbm_lattice.hpp"
+#include "lbm_phys.hpp"
+
+namespace lbm {
+
+template <int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_boundary(Grid g, const uint8_t* __restrict__ solid,
+                                                 uint8_t* __restrict__ bnd,
+                                                 uint32_t* __restrict__ wmask) {
+  constexpr int Q = Lat<DIM>::Q;
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  const Coord<DIM> c = decompose<DIM>(g, x);
+  uint32_t bits = 0;
+  if (!solid[x]) {
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      const int off = nb_offset<DIM>(g, c, E<DIM>(i, 0), E<DIM>(i, 1),
+                                     DIM == 3 ? E<DIM>(i, 2) : 0);
+      if (solid[(int)x + off]) bits |= 1u << i;
+    }
+  }
+  bnd[x] = bits ? 1 : 0;
+  wmask[x] = bits;
+}
+
+// numpy.gradient along one axis with unit spacing, edge_order=1.
+template <typename FT>
+__device__ __forceinline__ FT npgrad(const FT* __restrict__ f, uint32_t x, int i, int n,
+                                     int stride) {
+  if (n < 2) return FT(0);
+  if (i == 0) return (f[x + stride] - f[x]) / FT(1);
+  if (i == n - 1) return (f[x] - f[x - stride]) / FT(1);
+  return (f[x + stride] - f[x - stride]) / FT(2);
+}
+
+template <typename FT, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_normals_frac(Grid g, const FT* __restrict__ frac,
+                                                     const uint8_t* __restrict__ bnd,
+                                                     float* __restrict__ normal) {
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  const Coord<DIM> c = decompose<DIM>(g, x);
+  FT gr[DIM];
+  gr[0] = npgrad<FT>(frac, x, c.x, g.nx, g.sx);
+  gr[1] = npgrad<FT>(frac, x, c.y, g.ny, g.sy);
+  if constexpr (DIM == 3) gr[2] = npgrad<FT>(frac, x, c.z, g.nz, 1);
+  // Python's sum() starts from integer 0: ((0 + g0^2) + g1^2) + g2^2
+  FT s = gr[0] * gr[0];
+#pragma unroll
+  for (int d = 1; d < DIM; ++d) s = s + gr[d] * gr[d];
+  FT mag = (FT)sqrt(s);
+  if (mag < FT(1e-10)) mag = FT(1e-10);
+  const FT b = bnd[x] ? FT(1) : FT(0);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) normal[(long long)d * g.N + x] = (float)(((-gr[d]) / mag) * b);
+}
+
+template <int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_normals_discrete(Grid g,
+                                                         const uint8_t* __restrict__ solid,
+                                                         const uint8_t* __restrict__ bnd,
+                                                         float* __restrict__ normal) {
+  constexpr int Q = Lat<DIM>::Q;
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  double v[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) v[d] = 0.0;
+  if (bnd[x]) {
+    const Coord<DIM> c = decompose<DIM>(g, x);
+    int cnt = 0;
+    for (int i = 1; i < Q; ++i) {
+      const int px = c.x + E<DIM>(i, 0), py = c.y + E<DIM>(i, 1);
+      const int pz = DIM == 3 ? c.z + E<DIM>(i, 2) : 0;
+      if (px < 0 || px >= g.nx || py < 0 || py >= g.ny) continue;
+      if (DIM == 3 && (pz < 0 || pz >= g.nz)) continue;
+      const long long y = (long long)px * g.sx + (long long)py * g.sy + pz;
+      if (solid[y]) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) v[d] = v[d] - (double)E<DIM>(i, d);
+        ++cnt;
+      }
+    }
+    if (cnt > 0) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) v[d] = v[d] / (double)cnt;
+      double s = v[0] * v[0];
+#pragma unroll
+      for (int d = 1; d < DIM; ++d) s = s + v[d] * v[d];
+      const double nrm = sqrt(s);
+      if (nrm > 1e-10) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) v[d] = v[d] / nrm;
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) normal[(long long)d * g.N + x] = (float)v[d];
+}
+
+// Distance seed: 0 at boundary nodes, 100 elsewhere.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_dist_seed(long long N, const uint8_t* __restrict__ bnd,
+                                                  uint8_t* __restrict__ dist) {
+  const long long x = (long long)blockIdx.x * BS + threadIdx.x;
+  if (x >= N) return;
+  dist[x] = bnd[x] ? 0 : 100;
+}
+
+template <int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_dist_pass(Grid g, const uint8_t* __restrict__ din,
+                                                  uint8_t* __restrict__ dout) {
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  const Coord<DIM> c = decompose<DIM>(g, x);
+  int d = din[x];
+#pragma unroll
+  for (int ax = 0; ax < DIM; ++ax) {
+#pragma unroll
+    for (int s = -1; s <= 1; s += 2) {
+      const int off = nb_offset<DIM>(g, c, ax == 0 ? s : 0, ax == 1 ? s : 0, ax == 2 ? s : 0);
+      const int t = din[(int)x + off] + 1;
+      d = t < d ? t : d;
+    }
+  }
+  dout[x] = (uint8_t)(d > 100 ? 100 : d);
+}
+
+// Per-node geometry byte + partial-solid eps + S12 eligibility.
+template <typename T, typename FT, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_meta(Grid g, const uint8_t* __restrict__ solid,
+                                             const uint8_t* __restrict__ bnd,
+                                             const uint8_t* __restrict__ dist,
+                                             const FT* __restrict__ frac,
+                                             const float* __restrict__ normal,
+                                             uint8_t* __restrict__ meta, T* __restrict__ eps,
+                                             int wet) {
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  const bool s = solid[x] != 0;
+  uint8_t m = s ? M_SOLID : 0;
+  // boundary classification is only used by the wetting path
+  if (wet && bnd[x]) {
+    m |= M_BOUNDARY;
+    // |n_z| / max(|n|, 1e-10) > 0.5 in the run precision (fe_ac_solver.py:602-606)
+    T n[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) n[d] = (T)normal[(long long)d * g.N + x];
+    T nn = n[0] * n[0];
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) nn = nn + n[d] * n[d];
+    T mag = (T)sqrt(nn);
+    if (mag < T(1e-10)) mag = T(1e-10);
+    const T nz = n[DIM - 1] < T(0) ? -n[DIM - 1] : n[DIM - 1];
+    if (nz / mag > T(0.5)) m |= M_TOP;
+  }
+  T e = T(0);
+  if (frac) {
+    const FT fr = frac[x];
+    if (fr > FT(0.01) && fr < FT(0.99) && !s) {
+      m |= M_PARTIAL;
+      e = (T)fr;
+    }
+  }
+  int d = dist[x];
+  if (s) d = 100;
+  const int cls = d <= 11 ? d : 15;
+  m |= (uint8_t)(cls << 4);
+  meta[x] = m;
+  eps[x] = e;
+}
+
+// Topmost solid index per column (last axis), -1 if the column has none.
+template <int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_surface(Grid g, const uint8_t* __restrict__ solid,
+                                                int16_t* __restrict__ surf) {
+  const long long col = (long long)blockIdx.x * BS + threadIdx.x;
+  if constexpr (DIM == 3) {
+    if (col >= (long long)g.nx * g.ny) return;
+    int h = -1;
+    for (int z = g.nz - 1; z >= 0; --z)
+      if (solid[col * g.nz + z]) { h = z; break; }
+    surf[col] = (int16_t)h;
+  } else {
+    if (col >= g.nx) return;
+    int h = -1;
+    for (int y = g.ny - 1; y >= 0; --y)
+      if (solid[col * g.ny + y]) { h = y; break; }
+    surf[col] = (int16_t)h;
+  }
+}
+
+}  // namespace lbm

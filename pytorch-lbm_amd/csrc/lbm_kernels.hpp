@@ -1,0 +1,662 @@
+// Device kernels of the Allen-Cahn step.
+//
+// State between steps (pull scheme): Fin/Gin hold the post-collision sets
+// f*, g* of the previous step.  Every kernel that needs the post-stream
+// distributions of the current step pulls them node by node:
+//   fluid x   : f_i(x) = f*_i(x - e_i)                 (periodic, all axes)
+//   solid x   : f_i(x) = f*_opp(i)(x)                  (bounce-back)
+//   partial x : f_i(x) = (1-eps) f*_i(x-e_i) + eps f*_opp(i)(x)
+// and then applies the lagged mass factor c and the boundary-relax scale s,
+// so all writes of a step are node-local.  After init_fields the buffers hold
+// post-stream values instead ("local" form) and are read in place.
+//
+// Pass A (k_phi): phi = clamp(sum_i f_i, 0, 1), 0 at solid; fp64 block
+// partials of sum(phi); the last block to finish adds the partials in a fixed
+// order and writes the step's mass factor.
+// Pass B (k_collide): 1-ring stencil of phi with wetting ghosts, chemical
+// potential, geometric gradient correction, stress, forces, velocity, AC
+// source, both BGK collisions with Guo forcing; stores f*, g*, u.
+//
+// Arithmetic mirrors lbm/fe_ac_solver.py:380-639 operation by operation
+// (same association order, same scalar roundings); sums over directions use
+// the order of torch's CPU reduction (runs of 16, then run totals).
+#pragma once
+
+#include "lbm_lattice.hpp"
+#include "lbm_phys.hpp"
+
+namespace lbm {
+
+struct Ctrl {
+  double M0;
+  double mass[2];     // fp64 sum of the start-of-step phi, by step parity
+  double c[2];        // mass factor c_k, by step parity
+  unsigned int counter;
+  unsigned int counter2;
+  double stats[16];
+};
+
+template <typename T> __device__ __forceinline__ T clamp01(T v) {
+  return v < T(0) ? T(0) : (v > T(1) ? T(1) : v);
+}
+template <typename T> __device__ __forceinline__ T clamp_lo(T v, T lo) {
+  return v < lo ? lo : v;
+}
+
+template <int Q, typename T> __device__ __forceinline__ T qsum(const T (&a)[Q]) {
+  T tot = a[0];
+#pragma unroll
+  for (int i = 1; i < (Q < 16 ? Q : 16); ++i) tot = tot + a[i];
+  if constexpr (Q > 16) {
+    T r = a[16];
+#pragma unroll
+    for (int i = 17; i < Q; ++i) r = r + a[i];
+    tot = tot + r;
+  }
+  return tot;
+}
+
+// e_i . v, components summed in index order; zero components skipped (the
+// skipped terms are exact signed zeros in the reference).
+template <int DIM, typename T> __device__ __forceinline__ T edot(int i, const T* v) {
+  T acc = T(0);
+  bool first = true;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int e = E<DIM>(i, d);
+    if (e != 0) {
+      T t = e > 0 ? v[d] : -v[d];
+      acc = first ? t : acc + t;
+      first = false;
+    }
+  }
+  return acc;
+}
+
+template <int DIM, typename T> __device__ __forceinline__ T vsq(const T* v) {
+  T s = v[0] * v[0];
+#pragma unroll
+  for (int d = 1; d < DIM; ++d) s = s + v[d] * v[d];
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Pull of one distribution set at node x.
+// ---------------------------------------------------------------------------
+template <typename T, int DIM>
+__device__ __forceinline__ void pull(const T* __restrict__ F, long long N, uint32_t x,
+                                     const Coord<DIM>& c, const Grid& g, uint8_t m, T eps,
+                                     bool local, T (&f)[Lat<DIM>::Q]) {
+  constexpr int Q = Lat<DIM>::Q;
+  if (local) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = F[(long long)i * N + x];
+    return;
+  }
+  if (m & M_SOLID) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = F[(long long)OPP<DIM>(i) * N + x];
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
+                                   DIM == 3 ? -E<DIM>(i, 2) : 0);
+    f[i] = F[(long long)i * N + (long long)((int)x + off)];
+  }
+  if (m & M_PARTIAL) {
+    const T a = T(1) - eps;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = a * f[i] + eps * F[(long long)OPP<DIM>(i) * N + x];
+  }
+}
+
+// Mass factor and boundary relax of the previous step applied to a pulled f
+// (lbm/fe_ac_solver.py:582-591, 594-636).  phi_prev is that step's
+// start-of-step phi; the relaxed phi is written to *phi_relaxed when given.
+template <typename T, int DIM>
+__device__ __forceinline__ void scale_f(const Phys<T>& ph, T (&f)[Lat<DIM>::Q], T cfac,
+                                        uint8_t m, T phi_prev, T* phi_relaxed) {
+  constexpr int Q = Lat<DIM>::Q;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] = f[i] * cfac;
+  if (ph.s12_on && (m & M_TOP)) {
+    const T pc = clamp01(phi_prev * cfac);
+    if (pc > ph.thr_lo && pc < ph.thr_hi) {
+      const T pn = clamp01(pc + ph.s12_r * (ph.s12_t - pc));
+      const T tot = clamp_lo(qsum<Q>(f), T(1e-10));
+      T s = pn / tot;
+      s = s < T(0.5) ? T(0.5) : (s > T(2) ? T(2) : s);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) f[i] = f[i] * s;
+      if (phi_relaxed) *phi_relaxed = pn;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic block / grid reductions (fp64).
+// ---------------------------------------------------------------------------
+template <int BS> __device__ __forceinline__ double block_sum(double v, double* lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) r += lds[i];
+  }
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+// Writes this block's partial; the last block to arrive sums all partials in
+// index order and returns true in thread 0 with *total set.
+template <int BS>
+__device__ __forceinline__ bool grid_sum(double bsum, double* __restrict__ partials,
+                                         unsigned int* counter, double* lds, double* total) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = bsum;
+    __threadfence();
+    const unsigned int t = atomicAdd(counter, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  double v = 0.0;
+  const volatile double* vp = partials;
+  for (unsigned int i = threadIdx.x; i < gridDim.x; i += BS) v += vp[i];
+  const double tot = block_sum<BS>(v, lds);
+  if (threadIdx.x == 0) {
+    *total = tot;
+    *counter = 0u;
+  }
+  return threadIdx.x == 0;
+}
+
+// ---------------------------------------------------------------------------
+// init_fields: phi = clamp(phi_init), u, f = f_eq(phi, u), g = g_eq(0, u),
+// M0 = sum(phi) in fp64 (lbm/fe_ac_solver.py:239-261, 346-367).
+// ---------------------------------------------------------------------------
+template <typename T, int DIM>
+__device__ __forceinline__ void equilibria(const Phys<T>& ph, int i, T phi, T P, const T* u,
+                                           T usq_t, T* feq, T* geq) {
+  const T ec = edot<DIM>(i, u) / ph.cs2;
+  const T sq = (ec * ec) / T(2);
+  if (feq) *feq = phi * (ph.w[i] * (((T(1) + ec) + sq) - usq_t));
+  if (geq) *geq = P * ph.w[i] + ph.w[i] * ((ec + sq) - usq_t);
+}
+
+template <typename T, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __restrict__ phi_in,
+                                             const T* __restrict__ u_in, T* __restrict__ phi,
+                                             T* __restrict__ u, T* __restrict__ F,
+                                             T* __restrict__ G, double* __restrict__ partials,
+                                             Ctrl* ctrl) {
+  constexpr int Q = Lat<DIM>::Q;
+  __shared__ double lds[BS / 64];
+  const long long N = g.N;
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  double v = 0.0;
+  if (x < N) {
+    const T p = clamp01(phi_in[x]);
+    T uu[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) uu[d] = u_in ? u_in[(long long)d * N + x] : T(0);
+    phi[x] = p;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) u[(long long)d * N + x] = uu[d];
+    const T usq_t = vsq<DIM>(uu) / ph.two_cs2;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      T fe, ge;
+      equilibria<T, DIM>(ph, i, p, T(0), uu, usq_t, &fe, &ge);
+      F[(long long)i * N + x] = fe;
+      G[(long long)i * N + x] = ge;
+    }
+    v = (double)p;
+  }
+  const double b = block_sum<BS>(v, lds);
+  double tot;
+  if (grid_sum<BS>(b, partials, &ctrl->counter, lds, &tot)) ctrl->M0 = tot;
+}
+
+// ---------------------------------------------------------------------------
+// Pass A: start-of-step phi and its fp64 sum -> mass factor of this step.
+// ---------------------------------------------------------------------------
+template <typename T, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_phi(Grid g, Phys<T> ph, const T* __restrict__ Fin,
+                                            const T* __restrict__ phi_prev,
+                                            T* __restrict__ phi_out,
+                                            const uint8_t* __restrict__ meta,
+                                            const T* __restrict__ eps,
+                                            double* __restrict__ partials, Ctrl* ctrl,
+                                            int slot_prev, int slot_new, int local) {
+  constexpr int Q = Lat<DIM>::Q;
+  __shared__ double lds[BS / 64];
+  const long long N = g.N;
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  double v = 0.0;
+  if (x < N) {
+    const uint8_t m = meta[x];
+    const Coord<DIM> c = decompose<DIM>(g, x);
+    const T e = (m & M_PARTIAL) ? eps[x] : T(0);
+    T f[Q];
+    pull<T, DIM>(Fin, N, x, c, g, m, e, local != 0, f);
+    if (!local) {
+      const T cf = (T)ctrl->c[slot_prev];
+      scale_f<T, DIM>(ph, f, cf, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr);
+    }
+    T p = clamp01(qsum<Q>(f));
+    if (m & M_SOLID) p = T(0);
+    phi_out[x] = p;
+    v = (double)p;
+  }
+  const double b = block_sum<BS>(v, lds);
+  double m;
+  if (grid_sum<BS>(b, partials, &ctrl->counter, lds, &m)) {
+    ctrl->mass[slot_new] = m;
+    const double M0 = ctrl->M0;
+    ctrl->c[slot_new] = m > 1e-6 ? M0 / (m > 1e-6 ? m : 1e-6) : 1.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The 1-ring stencil with wetting ghosts (lbm/fe_ac_solver.py:296-320,
+// lbm/fe_wetting.py:418-446).  init_lap selects the stand-alone Laplacian of
+// init_fields (lbm/fe_ac_solver.py:282-294).
+// ---------------------------------------------------------------------------
+template <typename T, int DIM>
+__device__ __forceinline__ void stencil(const Phys<T>& ph, const Grid& g,
+                                        const T* __restrict__ phi, uint32_t x,
+                                        const Coord<DIM>& c, T p, uint32_t wm, const float* n32,
+                                        T* grad, T* lap, bool init_lap) {
+  constexpr int Q = Lat<DIM>::Q;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) grad[d] = T(0);
+  T l = T(0);
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    const int off = nb_offset<DIM>(g, c, E<DIM>(i, 0), E<DIM>(i, 1),
+                                   DIM == 3 ? E<DIM>(i, 2) : 0);
+    T psi = phi[(long long)((int)x + off)];
+    if (wm & (1u << i)) {
+      // n . e_i accumulated in fp32 from zero (lbm/fe_wetting.py:237-239)
+      float nde = 0.0f;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int e = E<DIM>(i, d);
+        if (e != 0) nde = nde + (e > 0 ? n32[d] : -n32[d]);
+        else nde = nde + 0.0f * n32[d];
+      }
+      if (ph.wet == WET_LINEAR) {
+        psi = p - (T)(nde * ph.hk32);
+      } else {
+        psi = p + ((T)(nde * ph.pck32) * p) * (T(1) - p);
+      }
+    }
+    const T diff = psi - p;
+    if (!init_lap) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        if (E<DIM>(i, d) != 0) grad[d] = grad[d] + ph.gcoef[i][d] * psi;
+      l = l + ph.lcoef[i] * diff;
+    } else {
+      l = l + (ph.linit[i] * diff) / ph.cs2;
+    }
+  }
+  *lap = l;
+}
+
+template <typename T>
+__device__ __forceinline__ T chem_pot(const Phys<T>& ph, T p, T lap, bool bnd) {
+  T mu = (((ph.four_beta * p) * (p - T(1))) * (p - T(0.5))) - ph.kappa * lap;
+  if (bnd) {
+    if (ph.wet == WET_LINEAR) mu = mu + ph.mu_wall_add;
+    else if (ph.wet == WET_QUADRATIC) mu = mu + (ph.phi_c * p) * (T(1) - p);
+  }
+  return mu;
+}
+
+// Geometric contact-angle correction at a wall-normal-carrying node
+// (lbm/fe_wetting.py:448-539); blend weight is 1 there (distance 0).
+template <typename T, int DIM>
+__device__ __forceinline__ void correct_grad(const Phys<T>& ph, T* gr, const float* n32,
+                                             T amp_node, bool amp_field) {
+  T n[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) n[d] = (T)n32[d];
+  T gn = n[0] * gr[0];
+#pragma unroll
+  for (int d = 1; d < DIM; ++d) gn = gn + n[d] * gr[d];
+  T gt[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) gt[d] = gr[d] - gn * n[d];
+  const T gtm = clamp_lo((T)sqrt(vsq<DIM>(gt)), T(1e-10));
+  T gd = ((-gtm) * ph.cos_t) / ph.sin_t;
+  if (amp_field) {
+    if (amp_node > T(1)) gd = gn + (gd - gn) * amp_node;
+  } else if (ph.amp_on) {
+    gd = gn + (gd - gn) * ph.amp;
+  }
+  const T gm = (T)sqrt(vsq<DIM>(gr));
+  if (gm > T(1e-6)) {
+    const T blend = T(1);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const T gc = gt[d] + gd * n[d];
+      gr[d] = gr[d] + blend * (gc - gr[d]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass B: everything from S2 to S8 at one node, then node-local stores.
+// ---------------------------------------------------------------------------
+template <typename T, int DIM> struct CollideArgs {
+  Grid g;
+  const T* __restrict__ Fin;
+  const T* __restrict__ Gin;
+  T* __restrict__ Fout;
+  T* __restrict__ Gout;
+  T* __restrict__ u;
+  const T* __restrict__ phi_cur;
+  const T* __restrict__ phi_prev;
+  const uint8_t* __restrict__ meta;
+  const uint32_t* __restrict__ wmask;
+  const float* __restrict__ normal;
+  const T* __restrict__ eps;
+  const T* __restrict__ ampf;
+  const Ctrl* ctrl;
+  int slot_prev;
+  int local;
+  int phiu_valid;
+};
+
+template <typename T, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, DIM> a, Phys<T> ph) {
+  constexpr int Q = Lat<DIM>::Q;
+  const Grid& g = a.g;
+  const long long N = g.N;
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= N) return;
+  const uint8_t m = a.meta[x];
+  const bool solid = m & M_SOLID;
+  const bool bnd = m & M_BOUNDARY;
+  const Coord<DIM> c = decompose<DIM>(g, x);
+  const T eps = (m & M_PARTIAL) ? a.eps[x] : T(0);
+
+  T f[Q], gg[Q];
+  pull<T, DIM>(a.Fin, N, x, c, g, m, eps, a.local != 0, f);
+  pull<T, DIM>(a.Gin, N, x, c, g, m, eps, a.local != 0, gg);
+  T pprev = T(0);
+  if (!a.local) {
+    if ((m & M_TOP) || a.phiu_valid) pprev = a.phi_prev[x];
+    scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot_prev], m, pprev, nullptr);
+  } else if (a.phiu_valid) {
+    pprev = a.phi_prev[x];
+  }
+
+  // S1/S2: phi, rho, stencil, mu, correction
+  const T p = a.phi_cur[x];
+  const T rho = p * ph.rho_l + (T(1) - p) * ph.rho_g;
+  const T rho_eff = rho < ph.rho_floor ? ph.rho_floor : rho;
+  float n32[DIM];
+  uint32_t wm = 0;
+  if (bnd && ph.wet != WET_NONE) {
+    wm = a.wmask[x];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) n32[d] = a.normal[(long long)d * N + x];
+  } else {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) n32[d] = 0.0f;
+  }
+  T gr[DIM], lap;
+  stencil<T, DIM>(ph, g, a.phi_cur, x, c, p, wm, n32, gr, &lap, false);
+  const T mu = chem_pot<T>(ph, p, lap, bnd && ph.wet != WET_NONE);
+  if (ph.corr_on && bnd) {
+    const T an = a.ampf ? a.ampf[x] : T(0);
+    correct_grad<T, DIM>(ph, gr, n32, an, a.ampf != nullptr);
+  }
+
+  // S3/S4: P, stress with the old velocity
+  const T P = qsum<Q>(gg);
+  T uo[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) uo[d] = a.u[(long long)d * N + x];
+  constexpr int NS = DIM == 3 ? 6 : 3;
+  T sig[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) sig[k] = T(0);
+  {
+    const T usq_t = vsq<DIM>(uo) / ph.two_cs2;
+    bool first[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) first[k] = true;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      T geq;
+      equilibria<T, DIM>(ph, i, T(0), P, uo, usq_t, nullptr, &geq);
+      const T gn = gg[i] - geq;
+      int k = 0;
+#pragma unroll
+      for (int aa = 0; aa < DIM; ++aa) {
+#pragma unroll
+        for (int bb = aa; bb < DIM; ++bb, ++k) {
+          const int ee = E<DIM>(i, aa) * E<DIM>(i, bb);
+          if (ee != 0) {
+            const T t = ee > 0 ? gn : -gn;
+            sig[k] = first[k] ? t : sig[k] + t;
+            first[k] = false;
+          }
+        }
+      }
+    }
+  }
+  auto S = [&](int aa, int bb) -> T {
+    if (aa > bb) { int t = aa; aa = bb; bb = t; }
+    // index of (aa,bb) in the packed upper triangle
+    int k = 0;
+    for (int r = 0; r < aa; ++r) k += DIM - r;
+    return sig[k + (bb - aa)];
+  };
+
+  // S5: forces
+  T F[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const T Fs = mu * gr[d];
+    const T Fp = (((-P) * ph.drho) * ph.cs2) * gr[d];
+    T sg = S(d, 0) * gr[0];
+#pragma unroll
+    for (int b = 1; b < DIM; ++b) sg = sg + S(d, b) * gr[b];
+    const T Fm = (ph.visc * sg) * ph.drho;
+    const T Fb = ph.body_on ? ph.gforce[d] * rho : T(0);
+    T t;
+    if (ph.stab == 0) t = ((Fs + Fp) + Fm) + Fb;
+    else if (ph.stab == 1) t = (Fs + Fp) + Fb;
+    else t = Fs + Fb;
+    F[d] = t;
+  }
+
+  // S6: velocity
+  T un[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    T acc = T(0);
+    bool first = true;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int e = E<DIM>(i, d);
+      if (e != 0) {
+        const T t = e > 0 ? gg[i] : -gg[i];
+        acc = first ? t : acc + t;
+        first = false;
+      }
+    }
+    un[d] = acc + (T(0.5) * F[d]) / rho_eff;
+    if (solid) un[d] = T(0);
+  }
+
+  // S7: interface source
+  T src[DIM];
+  {
+    T gm = (T)sqrt(vsq<DIM>(gr));
+    gm = clamp_lo(gm, T(1e-10));
+    const T lam = ((T(4) * p) * (T(1) - p)) / ph.xi;
+    const T sf = ph.sfac[dist_class(m)];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const T now = p * un[d];
+      const T prev = a.phiu_valid ? pprev * uo[d] : now;
+      const T dphiu = now - prev;
+      const T nv = gr[d] / gm;
+      T sh = ((ph.cs2 * lam) * nv) * ph.ac_scale;
+      if (ph.wet != WET_NONE) sh = sh * sf;
+      src[d] = dphiu + sh;
+    }
+  }
+
+  // S8: collisions
+  T ns, omg;
+  if (ph.var_tau) {
+    const T eta = ((p * ph.rho_l) * ph.tl_m) + (((T(1) - p) * ph.rho_g) * ph.tg_m);
+    const T rd = p * ph.rho_l + (T(1) - p) * ph.rho_g;
+    const T teff = T(0.5) + eta / rd;
+    ns = T(1) - T(0.5) / teff;
+    omg = T(1) / teff;
+  } else {
+    ns = ph.ns_c;
+    omg = ph.omg_c;
+  }
+  const T usq_n = vsq<DIM>(un) / ph.two_cs2;
+  const T rc = rho_eff * ph.cs2;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    T fe, ge;
+    equilibria<T, DIM>(ph, i, p, P, un, usq_n, &fe, &ge);
+    const T fsrc = ((ph.acoef * ph.w[i]) * edot<DIM>(i, src)) / ph.cs2;
+    const T fs = (f[i] - ph.om_f * (f[i] - fe)) + fsrc;
+    const T G = ((ns * ph.w[i]) * edot<DIM>(i, F)) / rc;
+    const T gs = (gg[i] - omg * (gg[i] - ge)) + G;
+    a.Fout[(long long)i * N + x] = fs;
+    a.Gout[(long long)i * N + x] = gs;
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) a.u[(long long)d * N + x] = un[d];
+}
+
+// ---------------------------------------------------------------------------
+// Observables (lazily materialised attributes of the solver object).
+// ---------------------------------------------------------------------------
+enum ObsKind : int {
+  OBS_PHI = 0, OBS_RHO = 1, OBS_P = 3, OBS_MU = 4, OBS_F = 5, OBS_G = 6,
+};
+
+template <typename T, int DIM> struct ObsArgs {
+  Grid g;
+  const T* __restrict__ Fsrc;   // F/G source buffer
+  const T* __restrict__ phi_cur;
+  const T* __restrict__ phi_prev;
+  const uint8_t* __restrict__ meta;
+  const uint32_t* __restrict__ wmask;
+  const float* __restrict__ normal;
+  const T* __restrict__ eps;
+  const Ctrl* ctrl;
+  T* __restrict__ out;
+  int kind;
+  int local;       // source buffer in post-stream form
+  int slot;        // mass factor slot of the step that produced phi_cur
+  int scaled;      // apply mass factor / relax (phi, f); 0 right after init
+  int init_lap;    // mu with the init_fields Laplacian
+};
+
+template <typename T, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, DIM> a, Phys<T> ph) {
+  constexpr int Q = Lat<DIM>::Q;
+  const Grid& g = a.g;
+  const long long N = g.N;
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= N) return;
+  const uint8_t m = a.meta[x];
+  const Coord<DIM> c = decompose<DIM>(g, x);
+  switch (a.kind) {
+    case OBS_PHI: {
+      T p = a.phi_cur[x];
+      if (a.scaled) {
+        const T cf = (T)a.ctrl->c[a.slot];
+        T pc = clamp01(p * cf);
+        if (ph.s12_on && (m & M_TOP) && pc > ph.thr_lo && pc < ph.thr_hi)
+          pc = clamp01(pc + ph.s12_r * (ph.s12_t - pc));
+        p = pc;
+      }
+      a.out[x] = p;
+      break;
+    }
+    case OBS_RHO: {
+      const T p = a.phi_cur[x];
+      a.out[x] = p * ph.rho_l + (T(1) - p) * ph.rho_g;
+      break;
+    }
+    case OBS_MU: {
+      const T p = a.phi_cur[x];
+      const bool bnd = (m & M_BOUNDARY) && ph.wet != WET_NONE;
+      float n32[DIM];
+      uint32_t wm = 0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) n32[d] = bnd ? a.normal[(long long)d * N + x] : 0.0f;
+      if (bnd) wm = a.wmask[x];
+      T gr[DIM], lap;
+      stencil<T, DIM>(ph, g, a.phi_cur, x, c, p, wm, n32, gr, &lap, a.init_lap != 0);
+      a.out[x] = chem_pot<T>(ph, p, lap, bnd);
+      break;
+    }
+    case OBS_P:
+    case OBS_F:
+    case OBS_G: {
+      const T e = (m & M_PARTIAL) ? a.eps[x] : T(0);
+      T f[Q];
+      pull<T, DIM>(a.Fsrc, N, x, c, g, m, e, a.local != 0, f);
+      if (a.kind == OBS_P) {
+        a.out[x] = qsum<Q>(f);
+        break;
+      }
+      if (a.kind == OBS_F && a.scaled && !a.local)
+        scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot], m, (m & M_TOP) ? a.phi_cur[x] : T(0),
+                        nullptr);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) a.out[(long long)i * N + x] = f[i];
+      break;
+    }
+  }
+}
+
+// Gradient and Laplacian of an arbitrary phi (the private operator the
+// reference's MMS script calls).
+template <typename T, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_gradlap(Grid g, Phys<T> ph, const T* __restrict__ phi,
+                                                const uint8_t* __restrict__ meta,
+                                                const uint32_t* __restrict__ wmask,
+                                                const float* __restrict__ normal,
+                                                T* __restrict__ grad, T* __restrict__ lap) {
+  const long long N = g.N;
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= N) return;
+  const Coord<DIM> c = decompose<DIM>(g, x);
+  const uint8_t m = meta ? meta[x] : 0;
+  const bool bnd = (m & M_BOUNDARY) && ph.wet != WET_NONE;
+  float n32[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) n32[d] = bnd ? normal[(long long)d * N + x] : 0.0f;
+  T gr[DIM], l;
+  stencil<T, DIM>(ph, g, phi, x, c, phi[x], bnd ? wmask[x] : 0u, n32, gr, &l, false);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) grad[(long long)d * N + x] = gr[d];
+  lap[x] = l;
+}
+
+}  // namespace lbm

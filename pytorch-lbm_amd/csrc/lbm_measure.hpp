@@ -1,0 +1,192 @@
+// Diagnostics reduced on the device (lbm/fe_measure.py:11-82 and the driver
+// metrics of scripts/run_thesis_sweep.py:185-206): extents of the liquid
+// (phi > 0.5 and not solid), volume (sum of phi with solid set to 0), C_max,
+// z_com (mean last-axis coordinate of the liquid nodes), NaN flag; and the
+// contact-line radius on the first fluid layer above each column's surface.
+// Each block writes one partial record; the last block combines the records
+// in block order, so results do not depend on scheduling.
+#pragma once
+
+#include "lbm_kernels.hpp"
+
+namespace lbm {
+
+struct MeasRec {
+  int mn[3], mx[3];
+  unsigned long long cnt, zsum;
+  double vol, cmax;
+  int nan;
+  int pad;
+};
+
+template <typename T, int DIM>
+__device__ __forceinline__ T phi_observed(const Phys<T>& ph, T p, uint8_t m, T cf, int scaled) {
+  if (!scaled) return p;
+  T pc = clamp01(p * cf);
+  if (ph.s12_on && (m & M_TOP) && pc > ph.thr_lo && pc < ph.thr_hi)
+    pc = clamp01(pc + ph.s12_r * (ph.s12_t - pc));
+  return pc;
+}
+
+template <int BS>
+__device__ void combine_rec(MeasRec* s, int n) {
+  // s[0..n) in LDS -> s[0], fixed tree order
+  for (int w = 1; w < n; w <<= 1) {
+    if ((int)threadIdx.x % (2 * w) == 0 && (int)threadIdx.x + w < n) {
+      MeasRec& a = s[threadIdx.x];
+      const MeasRec& b = s[threadIdx.x + w];
+      for (int d = 0; d < 3; ++d) {
+        a.mn[d] = b.mn[d] < a.mn[d] ? b.mn[d] : a.mn[d];
+        a.mx[d] = b.mx[d] > a.mx[d] ? b.mx[d] : a.mx[d];
+      }
+      a.cnt += b.cnt;
+      a.zsum += b.zsum;
+      a.vol += b.vol;
+      a.cmax = b.cmax > a.cmax ? b.cmax : a.cmax;
+      a.nan |= b.nan;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_measure(Grid g, Phys<T> ph, const T* __restrict__ phi,
+                                                const uint8_t* __restrict__ meta,
+                                                const Ctrl* ctrl, int slot, int scaled,
+                                                MeasRec* __restrict__ parts,
+                                                unsigned int* counter, double* out) {
+  __shared__ MeasRec s[BS];
+  __shared__ int is_last;
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  MeasRec r;
+  for (int d = 0; d < 3; ++d) { r.mn[d] = 0x7fffffff; r.mx[d] = -1; }
+  r.cnt = 0; r.zsum = 0; r.vol = 0.0; r.cmax = 0.0; r.nan = 0; r.pad = 0;
+  if (x < g.N) {
+    const uint8_t m = meta[x];
+    const T cf = scaled ? (T)ctrl->c[slot] : T(1);
+    T p = phi_observed<T, DIM>(ph, phi[x], m, cf, scaled);
+    if (p != p) r.nan = 1;
+    if (m & M_SOLID) p = T(0);
+    r.vol = (double)p;
+    r.cmax = (double)p;
+    if (p > T(0.5)) {
+      const Coord<DIM> c = decompose<DIM>(g, x);
+      const int cc[3] = {c.x, c.y, c.z};
+      for (int d = 0; d < DIM; ++d) { r.mn[d] = cc[d]; r.mx[d] = cc[d]; }
+      r.cnt = 1;
+      r.zsum = (unsigned long long)cc[DIM - 1];
+    }
+  }
+  s[threadIdx.x] = r;
+  __syncthreads();
+  combine_rec<BS>(s, BS);
+  if (threadIdx.x == 0) {
+    parts[blockIdx.x] = s[0];
+    __threadfence();
+    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  // last block: fold partial records in block order, BS at a time
+  MeasRec acc;
+  for (int d = 0; d < 3; ++d) { acc.mn[d] = 0x7fffffff; acc.mx[d] = -1; }
+  acc.cnt = 0; acc.zsum = 0; acc.vol = 0.0; acc.cmax = 0.0; acc.nan = 0; acc.pad = 0;
+  for (unsigned int base = 0; base < gridDim.x; base += BS) {
+    __syncthreads();
+    if (base + threadIdx.x < gridDim.x) {
+      const volatile MeasRec* vp = parts + base + threadIdx.x;
+      MeasRec t;
+      for (int d = 0; d < 3; ++d) { t.mn[d] = vp->mn[d]; t.mx[d] = vp->mx[d]; }
+      t.cnt = vp->cnt; t.zsum = vp->zsum; t.vol = vp->vol; t.cmax = vp->cmax;
+      t.nan = vp->nan; t.pad = 0;
+      s[threadIdx.x] = t;
+    } else {
+      MeasRec t;
+      for (int d = 0; d < 3; ++d) { t.mn[d] = 0x7fffffff; t.mx[d] = -1; }
+      t.cnt = 0; t.zsum = 0; t.vol = 0.0; t.cmax = 0.0; t.nan = 0; t.pad = 0;
+      s[threadIdx.x] = t;
+    }
+    __syncthreads();
+    combine_rec<BS>(s, BS);
+    if (threadIdx.x == 0) {
+      const MeasRec& b = s[0];
+      for (int d = 0; d < 3; ++d) {
+        acc.mn[d] = b.mn[d] < acc.mn[d] ? b.mn[d] : acc.mn[d];
+        acc.mx[d] = b.mx[d] > acc.mx[d] ? b.mx[d] : acc.mx[d];
+      }
+      acc.cnt += b.cnt; acc.zsum += b.zsum; acc.vol += b.vol;
+      acc.cmax = b.cmax > acc.cmax ? b.cmax : acc.cmax;
+      acc.nan |= b.nan;
+    }
+  }
+  if (threadIdx.x == 0) {
+    double D[3] = {0, 0, 0};
+    if (acc.cnt > 0)
+      for (int d = 0; d < DIM; ++d) D[d] = (double)(acc.mx[d] - acc.mn[d] + 1);
+    out[0] = D[0];
+    out[1] = D[1];
+    out[2] = DIM == 3 ? D[2] : 0.0;
+    out[3] = (acc.cnt > 0 && D[1] > 0) ? D[0] / D[1] : 0.0;
+    out[4] = acc.vol;
+    out[5] = acc.cmax;
+    out[6] = acc.cnt > 0 ? (double)acc.zsum / (double)acc.cnt : 0.0;
+    out[7] = (double)acc.cnt;
+    out[8] = acc.vol;
+    out[9] = (double)acc.nan;
+    *counter = 0u;
+  }
+}
+
+// Contact-line radius: one thread per column.
+template <typename T, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_contact(Grid g, Phys<T> ph, const T* __restrict__ phi,
+                                                const uint8_t* __restrict__ meta,
+                                                const int16_t* __restrict__ surf,
+                                                const Ctrl* ctrl, int slot, int scaled,
+                                                double cx, double cy,
+                                                double* __restrict__ parts,
+                                                unsigned int* counter, double* out) {
+  __shared__ double s[BS];
+  __shared__ int is_last;
+  const long long col = (long long)blockIdx.x * BS + threadIdx.x;
+  const long long ncol = (long long)g.nx * g.ny;
+  double r2 = -1.0;
+  if (DIM == 3 && col < ncol) {
+    const int h = surf[col];
+    const int z = h + 1;
+    if (z < g.nz) {
+      const long long x = col * g.nz + z;
+      const uint8_t m = meta[x];
+      const T cf = scaled ? (T)ctrl->c[slot] : T(1);
+      const T p = phi_observed<T, DIM>(ph, phi[x], m, cf, scaled);
+      if (!(m & M_SOLID) && p > T(0.5)) {
+        const double dx = (double)(col / g.ny) - cx, dy = (double)(col % g.ny) - cy;
+        r2 = dx * dx + dy * dy;
+      }
+    }
+  }
+  s[threadIdx.x] = r2;
+  __syncthreads();
+  for (int w = BS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && s[threadIdx.x + w] > s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    parts[blockIdx.x] = s[0];
+    __threadfence();
+    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double best = -1.0;
+    const volatile double* vp = parts;
+    for (unsigned int i = 0; i < gridDim.x; ++i) best = vp[i] > best ? vp[i] : best;
+    *out = best < 0 ? 0.0 : sqrt(best);
+    *counter = 0u;
+  }
+}
+
+}  // namespace lbm

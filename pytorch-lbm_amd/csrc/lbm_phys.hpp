@@ -1,0 +1,41 @@
+// Physics constants of one solver, prepared once on the host in the run
+// precision T so that every scalar enters the device arithmetic exactly as
+// the reference's tensor ops see it (a Python float meeting a float32 tensor
+// is rounded to float32 first; values the reference derives with tensor
+// arithmetic, such as w/cs2 or 1/tau_eff, are derived here with T arithmetic).
+#pragma once
+
+#include "lbm_lattice.hpp"
+
+namespace lbm {
+
+enum WetMode : int { WET_NONE = 0, WET_LINEAR = 1, WET_QUADRATIC = 2 };
+
+template <typename T> struct Phys {
+  T w[19];            // T(w32_i)
+  T gcoef[19][3];     // (w_i / cs2) * e_id      (lbm/fe_ac_solver.py:306,318)
+  T lcoef[19];        // 2 * (w_i / cs2)         (lbm/fe_ac_solver.py:319)
+  T linit[19];        // 2 * w_i                 (lbm/fe_ac_solver.py:293)
+  T cs2, two_cs2;
+  T four_beta, kappa, rho_l, rho_g, rho_floor, drho, visc, xi, ac_scale;
+  T mu_wall_add;      // linear mode: mu_wall * mu_wall_scale
+  T phi_c;            // quadratic mode
+  float hk32, pck32;  // h/kappa*ghost_scale, phi_c/kappa (fp32 n.e products)
+  T cos_t, sin_t, amp;
+  int amp_on;         // scalar alpha > 1
+  int corr_on;        // geometric gradient correction active
+  T sfac[16];         // 1 - sharpen_reduce * wall_weight per distance class
+  T acoef, om_f;      // 1 - 0.5/tau_f, 1/tau_f
+  T ns_c, omg_c;      // 1 - 0.5/tau_eff, 1/tau_eff when tau_eff is constant
+  int var_tau;
+  T tl_m, tg_m;       // tau_l - 0.5, tau_g - 0.5
+  T gforce[3];
+  int body_on;
+  int stab;           // 0 fakhari, 1 liang, 2 none
+  int wet;            // WetMode
+  int s12_on;
+  T s12_r, s12_t;
+  T thr_lo, thr_hi;   // 0.15, 0.85
+};
+
+}  // namespace lbm
