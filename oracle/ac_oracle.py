@@ -248,6 +248,25 @@ def make_geometry(p: Params, solid, frac=None):
 
 
 # ---------------------------------------------------------------------------
+# 16-bit storage model (the build's precision extension; no reference
+# counterpart): f* is stored as f*_i(y) - w_i phi(y + e_i) and g* as is, both
+# rounded to nearest-even in bf16 or fp16, then reconstructed in fp32.
+# ---------------------------------------------------------------------------
+def round_bf16(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = a.view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    out = r.astype(np.uint32).view(np.float32)
+    return np.where(np.isnan(a), a, out)
+
+
+def round16(a, kind):
+    if kind == 'fp16':
+        return np.asarray(a, np.float32).astype(np.float16).astype(np.float32)
+    return round_bf16(a)
+
+
+# ---------------------------------------------------------------------------
 # The solver state machine
 # ---------------------------------------------------------------------------
 class OracleSolver:
@@ -261,8 +280,11 @@ class OracleSolver:
     def __init__(self, p: Params, dtype=np.float32, stab_mode='fakhari',
                  boundary_relax=0.3, ghost_scale=None, sharpen_reduce=0.8,
                  mu_wall_scale=1.0, geometric_wetting=False,
-                 geo_amplification=1.0, ac_scale=1.0):
+                 geo_amplification=1.0, ac_scale=1.0, storage=None):
         self.p = p
+        self.storage = storage          # None | 'bf16' | 'fp16' (fp32 compute)
+        if storage is not None:
+            dtype = np.float32
         self.dt = np.dtype(dtype)
         self.stab_mode = 'fakhari' if stab_mode == 'auto' else stab_mode
         self.boundary_relax = boundary_relax
@@ -455,6 +477,10 @@ class OracleSolver:
         self.mu = self.chem_pot(phi, lap)
         self.f = self.f_eq(phi, u)
         self.g = self.g_eq(self.P, u)
+        if self.storage is not None:
+            wv = self._wv()
+            self.f = round16(self.f - wv * phi[None], self.storage) + wv * phi[None]
+            self.g = round16(self.g, self.storage)
         self.phi_u_prev = None
 
     # -- the step -------------------------------------------------------------
@@ -552,6 +578,13 @@ class OracleSolver:
         geq = self.g_eq(P, u)
         omg = (1.0 / teff).astype(dt)
         g = ((self.g - omg[None] * (self.g - geq)) + G).astype(dt)
+        if self.storage is not None:
+            # slot (i, y) keeps f*_i(y) - w_i phi(y + e_i); the pulling node x =
+            # y + e_i adds back w_i phi(x): the same value, rounded once.
+            for i in range(self.Q):
+                ref = self.w[i] * neighbor(phi, self.e[i])
+                f[i] = round16(f[i] - ref, self.storage) + ref
+            g = round16(g, self.storage)
         st.update(P=P, sigma=sig, F=F, u=u, Fsrc=Fsrc, fstar=f.copy(), gstar=g.copy())
         # S9-S10b: stream (pull from x - e_i), bounce-back, volume penalization
         fs, gs = f, g

@@ -45,7 +45,8 @@ struct lbm_solver {
   int dim = 3, Q = 19;
   Grid g{};
   long long N = 0;
-  size_t esz = 4;            // bytes per stored/computed element
+  size_t esz = 4;            // bytes per computed / field element
+  size_t dsz = 4;            // bytes per stored distribution element
   int wet = WET_NONE;
   double hk = 0.0, mu_wall = 0.0, pck = 0.0, target = 0.0;
   bool has_target = false;
@@ -201,8 +202,8 @@ int lbm_create(const lbm_params* pin, lbm_solver** out) {
   if (p.nx < 2 || p.ny < 2 || p.nz < 1)
     return fail(LBM_EINVAL, "grid must be at least 2 x 2 (x 2 in 3D)");
   if (p.nz == 2) return fail(LBM_EINVAL, "nz must be 1 (2D) or >= 3");
-  if (p.precision != LBM_F32 && p.precision != LBM_F64)
-    return fail(LBM_EINVAL, "precision not supported by this entry point");
+  if (p.precision < LBM_F32 || p.precision > LBM_F16)
+    return fail(LBM_EINVAL, "unknown precision");
   if (!(p.tau_l > 0.5) || !(p.tau_g > 0.5))
     return fail(LBM_EINVAL, "tau_l and tau_g must exceed 0.5");
   if (p.stab_mode < 0 || p.stab_mode > 2) return fail(LBM_EINVAL, "bad stab_mode");
@@ -215,6 +216,7 @@ int lbm_create(const lbm_params* pin, lbm_solver** out) {
   s->g = make_grid(p.nx, p.ny, p.nz);
   s->N = N;
   s->esz = p.precision == LBM_F64 ? 8 : 4;
+  s->dsz = p.precision == LBM_F64 ? 8 : (p.precision == LBM_F32 ? 4 : 2);
   // wetting mode (lbm/fe_ac_solver.py:223-236, lbm/fe_wetting.py:85-115)
   if (p.theta_eq != 90.0 && p.kappa > 0) {
     s->wet = (p.theta_eq > 130.0 || p.theta_eq < 50.0) ? WET_LINEAR : WET_QUADRATIC;
@@ -245,8 +247,8 @@ int lbm_create(const lbm_params* pin, lbm_solver** out) {
     if (e == hipSuccess) e = hipMemset((ptr), 0, (bytes));        \
   }
   for (int b = 0; b < 2; ++b) {
-    ALLOC(s->Fb[b], Q * N * es);
-    ALLOC(s->Gb[b], Q * N * es);
+    ALLOC(s->Fb[b], Q * N * s->dsz);
+    ALLOC(s->Gb[b], Q * N * s->dsz);
     ALLOC(s->phi[b], N * es);
     ALLOC(s->dist[b], N);
   }
@@ -320,6 +322,16 @@ static int set_geometry_impl(lbm_solver* s, const void* frac, int fdt, hipStream
   return LBM_OK;
 }
 
+// (compute type, storage type, dimension) of a solver
+#define DISPATCH3(s, CALL)                                                           \
+  ((s)->p.precision == LBM_F64                                                       \
+       ? (((s)->dim == 3) ? CALL(double, double, 3) : CALL(double, double, 2))       \
+   : (s)->p.precision == LBM_F32                                                     \
+       ? (((s)->dim == 3) ? CALL(float, float, 3) : CALL(float, float, 2))           \
+   : (s)->p.precision == LBM_BF16                                                    \
+       ? (((s)->dim == 3) ? CALL(float, __hip_bfloat16, 3)                           \
+                          : CALL(float, __hip_bfloat16, 2))                          \
+       : (((s)->dim == 3) ? CALL(float, __half, 3) : CALL(float, __half, 2)))
 #define DISPATCH(s, CALL)                                                  \
   (((s)->esz == 8) ? (((s)->dim == 3) ? CALL(double, 3) : CALL(double, 2)) \
                    : (((s)->dim == 3) ? CALL(float, 3) : CALL(float, 2)))
@@ -376,12 +388,12 @@ int lbm_set_amplification_field(lbm_solver* s, const void* amp_dev, void* stream
   return LBM_OK;
 }
 
-template <typename T, int DIM>
+template <typename T, typename S, int DIM>
 static int init_impl(lbm_solver* s, const void* phi, const void* u, hipStream_t st) {
   const Phys<T> ph = make_phys<T, DIM>(s);
-  k_init<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(s->g, ph, (const T*)phi, (const T*)u,
-                                                (T*)s->phi[0], (T*)s->u, (T*)s->Fb[0],
-                                                (T*)s->Gb[0], s->partials, s->ctrl);
+  k_init<T, S, DIM, BS><<<s->nblocks, BS, 0, st>>>(s->g, ph, (const T*)phi, (const T*)u,
+                                                   (T*)s->phi[0], (T*)s->u, (S*)s->Fb[0],
+                                                   (S*)s->Gb[0], s->partials, s->ctrl);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -394,8 +406,8 @@ int lbm_init_fields(lbm_solver* s, const void* phi_dev, const void* u_dev, void*
     int rc = lbm_set_geometry(s, s->solid, nullptr, LBM_FRAC_NONE, stream);
     if (rc) return rc;
   }
-#define CALL(T, D) init_impl<T, D>(s, phi_dev, u_dev, (hipStream_t)stream)
-  int rc = DISPATCH(s, CALL);
+#define CALL(T, S, D) init_impl<T, S, D>(s, phi_dev, u_dev, (hipStream_t)stream)
+  int rc = DISPATCH3(s, CALL);
 #undef CALL
   if (rc) return rc;
   s->inited = true;
@@ -407,23 +419,23 @@ int lbm_init_fields(lbm_solver* s, const void* phi_dev, const void* u_dev, void*
   return LBM_OK;
 }
 
-template <typename T, int DIM>
+template <typename T, typename S, int DIM>
 static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev = nullptr) {
   const Phys<T> ph = make_phys<T, DIM>(s);
   for (int it = 0; it < nsteps; ++it) {
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 0], st));
     const int src = s->cur, dst = s->cur ^ 1;
     const int pslot = (int)(s->k & 1), nslot = (int)((s->k + 1) & 1);
-    k_phi<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(
-        s->g, ph, (const T*)s->Fb[src], (const T*)s->phi[pslot], (T*)s->phi[nslot], s->meta,
+    k_phi<T, S, DIM, BS><<<s->nblocks, BS, 0, st>>>(
+        s->g, ph, (const S*)s->Fb[src], (const T*)s->phi[pslot], (T*)s->phi[nslot], s->meta,
         (const T*)s->eps, s->partials, s->ctrl, pslot, nslot, s->local ? 1 : 0);
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 1], st));
-    CollideArgs<T, DIM> a;
+    CollideArgs<T, S, DIM> a;
     a.g = s->g;
-    a.Fin = (const T*)s->Fb[src];
-    a.Gin = (const T*)s->Gb[src];
-    a.Fout = (T*)s->Fb[dst];
-    a.Gout = (T*)s->Gb[dst];
+    a.Fin = (const S*)s->Fb[src];
+    a.Gin = (const S*)s->Gb[src];
+    a.Fout = (S*)s->Fb[dst];
+    a.Gout = (S*)s->Gb[dst];
     a.u = (T*)s->u;
     a.phi_cur = (const T*)s->phi[nslot];
     a.phi_prev = (const T*)s->phi[pslot];
@@ -436,7 +448,7 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     a.slot_prev = pslot;
     a.local = s->local ? 1 : 0;
     a.phiu_valid = s->phiu_valid ? 1 : 0;
-    k_collide<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(a, ph);
+    k_collide<T, S, DIM, BS><<<s->nblocks, BS, 0, st>>>(a, ph);
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 2], st));
     s->prev_local = s->local;
     s->local = false;
@@ -452,8 +464,8 @@ int lbm_step(lbm_solver* s, int nsteps, void* stream) {
   if (!s) return fail(LBM_EINVAL, "null solver");
   if (!s->inited) return fail(LBM_ESTATE, "step() before init_fields()");
   if (nsteps < 0) return fail(LBM_EINVAL, "negative step count");
-#define CALL(T, D) step_impl<T, D>(s, nsteps, (hipStream_t)stream)
-  return DISPATCH(s, CALL);
+#define CALL(T, S, D) step_impl<T, S, D>(s, nsteps, (hipStream_t)stream)
+  return DISPATCH3(s, CALL);
 #undef CALL
 }
 
@@ -512,8 +524,8 @@ int lbm_profile_steps(lbm_solver* s, int nsteps, void* stream, double* out) {
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) rc = fail(LBM_EHIP, "hipEventCreate failed");
   if (rc == LBM_OK) {
-#define CALL(T, D) step_impl<T, D>(s, nsteps, st, ev.data())
-    rc = DISPATCH(s, CALL);
+#define CALL(T, S, D) step_impl<T, S, D>(s, nsteps, st, ev.data())
+    rc = DISPATCH3(s, CALL);
 #undef CALL
   }
   if (rc == LBM_OK && hipStreamSynchronize(st) != hipSuccess)
@@ -584,13 +596,14 @@ __global__ void k_geom_obs(long long N, const uint8_t* __restrict__ meta,
   }
 }
 
-template <typename T, int DIM>
+template <typename T, typename S, int DIM>
 static int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st) {
   const Phys<T> ph = make_phys<T, DIM>(s);
-  ObsArgs<T, DIM> a;
+  ObsArgs<T, S, DIM> a;
   std::memset(&a, 0, sizeof(a));
   a.g = s->g;
   a.phi_cur = (const T*)s->phi[s->k & 1];
+  a.pref = a.phi_cur;
   a.phi_prev = (const T*)s->phi[(s->k + 1) & 1];
   a.meta = s->meta;
   a.wmask = s->wmask;
@@ -615,15 +628,15 @@ static int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st) {
       a.kind = OBS_MU;
       a.init_lap = s->k == 0 ? 1 : 0;
       break;
-    case LBM_FIELD_F: a.kind = OBS_F; a.Fsrc = (const T*)s->Fb[s->cur]; break;
-    case LBM_FIELD_G: a.kind = OBS_G; a.Fsrc = (const T*)s->Gb[s->cur]; break;
+    case LBM_FIELD_F: a.kind = OBS_F; a.Fsrc = (const S*)s->Fb[s->cur]; break;
+    case LBM_FIELD_G: a.kind = OBS_G; a.Fsrc = (const S*)s->Gb[s->cur]; break;
     case LBM_FIELD_P:
       if (s->k == 0) {
         HIPCHK(hipMemsetAsync(dst, 0, N * s->esz, st));
         return LBM_OK;
       }
       a.kind = OBS_P;
-      a.Fsrc = (const T*)s->Gb[s->cur ^ 1];
+      a.Fsrc = (const S*)s->Gb[s->cur ^ 1];
       a.local = s->prev_local ? 1 : 0;
       break;
     case LBM_FIELD_WALL_NORMAL:
@@ -637,7 +650,7 @@ static int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st) {
     default:
       return fail(LBM_EINVAL, "unknown field");
   }
-  k_observe<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(a, ph);
+  k_observe<T, S, DIM, BS><<<s->nblocks, BS, 0, st>>>(a, ph);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -647,8 +660,8 @@ int lbm_get_field(lbm_solver* s, int which, void* dst_dev, void* stream) {
   if (!s->geom) return fail(LBM_ESTATE, "geometry not set");
   const bool geo = which >= LBM_FIELD_IS_BOUNDARY;
   if (!geo && !s->inited) return fail(LBM_ESTATE, "fields not initialised");
-#define CALL(T, D) get_field_impl<T, D>(s, which, dst_dev, (hipStream_t)stream)
-  return DISPATCH(s, CALL);
+#define CALL(T, S, D) get_field_impl<T, S, D>(s, which, dst_dev, (hipStream_t)stream)
+  return DISPATCH3(s, CALL);
 #undef CALL
 }
 

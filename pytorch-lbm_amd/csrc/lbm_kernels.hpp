@@ -22,6 +22,11 @@
 // the order of torch's CPU reduction (runs of 16, then run totals).
 #pragma once
 
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <type_traits>
+
 #include "lbm_lattice.hpp"
 #include "lbm_phys.hpp"
 
@@ -34,6 +39,33 @@ struct Ctrl {
   unsigned int counter;
   unsigned int counter2;
   double stats[16];
+};
+
+// Distribution storage: fp32/fp64 as is; bf16/fp16 with round-to-nearest-even
+// conversion, collided in fp32.
+template <typename S> struct Stor {
+  template <typename T> __device__ static __forceinline__ T ld(const S* p, long long i) {
+    return (T)p[i];
+  }
+  template <typename T> __device__ static __forceinline__ void st(S* p, long long i, T v) {
+    p[i] = (S)v;
+  }
+};
+template <> struct Stor<__hip_bfloat16> {
+  template <typename T> __device__ static __forceinline__ T ld(const __hip_bfloat16* p, long long i) {
+    return (T)__bfloat162float(p[i]);
+  }
+  template <typename T> __device__ static __forceinline__ void st(__hip_bfloat16* p, long long i, T v) {
+    p[i] = __float2bfloat16((float)v);
+  }
+};
+template <> struct Stor<__half> {
+  template <typename T> __device__ static __forceinline__ T ld(const __half* p, long long i) {
+    return (T)__half2float(p[i]);
+  }
+  template <typename T> __device__ static __forceinline__ void st(__half* p, long long i, T v) {
+    p[i] = __float2half((float)v);
+  }
 };
 
 template <typename T> __device__ __forceinline__ T clamp01(T v) {
@@ -82,32 +114,64 @@ template <int DIM, typename T> __device__ __forceinline__ T vsq(const T* v) {
 
 // ---------------------------------------------------------------------------
 // Pull of one distribution set at node x.
+//
+// DEV (16-bit f storage): slot (i, y) holds f*_i(y) - w_i phi_ref(y + e_i),
+// phi_ref being the start-of-step phi of the step that wrote it, so a fluid
+// pull adds back w_i phi_ref(x) of the pulling node itself; bounce-back and
+// penalization slots (i, x) read at solid/partial x add w_i phi_ref(x - e_i).
 // ---------------------------------------------------------------------------
-template <typename T, int DIM>
-__device__ __forceinline__ void pull(const T* __restrict__ F, long long N, uint32_t x,
+template <typename T, typename S, int DIM, bool DEV>
+__device__ __forceinline__ void pull(const S* __restrict__ F, long long N, uint32_t x,
                                      const Coord<DIM>& c, const Grid& g, uint8_t m, T eps,
-                                     bool local, T (&f)[Lat<DIM>::Q]) {
+                                     bool local, const T* __restrict__ pref, const Phys<T>& ph,
+                                     T (&f)[Lat<DIM>::Q]) {
   constexpr int Q = Lat<DIM>::Q;
+  using L = Stor<S>;
   if (local) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = F[(long long)i * N + x];
+    for (int i = 0; i < Q; ++i) f[i] = L::template ld<T>(F, (long long)i * N + x);
+    if constexpr (DEV) {
+      const T r = pref[x];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.w[i] * r;
+    }
     return;
   }
   if (m & M_SOLID) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = F[(long long)OPP<DIM>(i) * N + x];
+    for (int i = 0; i < Q; ++i) {
+      f[i] = L::template ld<T>(F, (long long)OPP<DIM>(i) * N + x);
+      if constexpr (DEV) {
+        const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
+                                       DIM == 3 ? -E<DIM>(i, 2) : 0);
+        f[i] = f[i] + ph.w[i] * pref[(long long)((int)x + off)];
+      }
+    }
     return;
   }
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
                                    DIM == 3 ? -E<DIM>(i, 2) : 0);
-    f[i] = F[(long long)i * N + (long long)((int)x + off)];
+    f[i] = L::template ld<T>(F, (long long)i * N + (long long)((int)x + off));
+  }
+  if constexpr (DEV) {
+    const T r = pref[x];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.w[i] * r;
   }
   if (m & M_PARTIAL) {
     const T a = T(1) - eps;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = a * f[i] + eps * F[(long long)OPP<DIM>(i) * N + x];
+    for (int i = 0; i < Q; ++i) {
+      T o = L::template ld<T>(F, (long long)OPP<DIM>(i) * N + x);
+      if constexpr (DEV) {
+        const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
+                                       DIM == 3 ? -E<DIM>(i, 2) : 0);
+        o = o + ph.w[i] * pref[(long long)((int)x + off)];
+      }
+      f[i] = a * f[i] + eps * o;
+    }
   }
 }
 
@@ -191,12 +255,13 @@ __device__ __forceinline__ void equilibria(const Phys<T>& ph, int i, T phi, T P,
   if (geq) *geq = P * ph.w[i] + ph.w[i] * ((ec + sq) - usq_t);
 }
 
-template <typename T, int DIM, int BS>
+template <typename T, typename S, int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __restrict__ phi_in,
                                              const T* __restrict__ u_in, T* __restrict__ phi,
-                                             T* __restrict__ u, T* __restrict__ F,
-                                             T* __restrict__ G, double* __restrict__ partials,
+                                             T* __restrict__ u, S* __restrict__ F,
+                                             S* __restrict__ G, double* __restrict__ partials,
                                              Ctrl* ctrl) {
+  constexpr bool DEV = !std::is_same<T, S>::value;
   constexpr int Q = Lat<DIM>::Q;
   __shared__ double lds[BS / 64];
   const long long N = g.N;
@@ -215,8 +280,9 @@ __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __rest
     for (int i = 0; i < Q; ++i) {
       T fe, ge;
       equilibria<T, DIM>(ph, i, p, T(0), uu, usq_t, &fe, &ge);
-      F[(long long)i * N + x] = fe;
-      G[(long long)i * N + x] = ge;
+      if constexpr (DEV) fe = fe - ph.w[i] * p;
+      Stor<S>::st(F, (long long)i * N + x, fe);
+      Stor<S>::st(G, (long long)i * N + x, ge);
     }
     v = (double)p;
   }
@@ -228,8 +294,8 @@ __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __rest
 // ---------------------------------------------------------------------------
 // Pass A: start-of-step phi and its fp64 sum -> mass factor of this step.
 // ---------------------------------------------------------------------------
-template <typename T, int DIM, int BS>
-__global__ __launch_bounds__(BS) void k_phi(Grid g, Phys<T> ph, const T* __restrict__ Fin,
+template <typename T, typename S, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_phi(Grid g, Phys<T> ph, const S* __restrict__ Fin,
                                             const T* __restrict__ phi_prev,
                                             T* __restrict__ phi_out,
                                             const uint8_t* __restrict__ meta,
@@ -246,7 +312,8 @@ __global__ __launch_bounds__(BS) void k_phi(Grid g, Phys<T> ph, const T* __restr
     const Coord<DIM> c = decompose<DIM>(g, x);
     const T e = (m & M_PARTIAL) ? eps[x] : T(0);
     T f[Q];
-    pull<T, DIM>(Fin, N, x, c, g, m, e, local != 0, f);
+    pull<T, S, DIM, !std::is_same<T, S>::value>(Fin, N, x, c, g, m, e, local != 0, phi_prev,
+                                                ph, f);
     if (!local) {
       const T cf = (T)ctrl->c[slot_prev];
       scale_f<T, DIM>(ph, f, cf, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr);
@@ -357,12 +424,12 @@ __device__ __forceinline__ void correct_grad(const Phys<T>& ph, T* gr, const flo
 // ---------------------------------------------------------------------------
 // Pass B: everything from S2 to S8 at one node, then node-local stores.
 // ---------------------------------------------------------------------------
-template <typename T, int DIM> struct CollideArgs {
+template <typename T, typename S, int DIM> struct CollideArgs {
   Grid g;
-  const T* __restrict__ Fin;
-  const T* __restrict__ Gin;
-  T* __restrict__ Fout;
-  T* __restrict__ Gout;
+  const S* __restrict__ Fin;
+  const S* __restrict__ Gin;
+  S* __restrict__ Fout;
+  S* __restrict__ Gout;
   T* __restrict__ u;
   const T* __restrict__ phi_cur;
   const T* __restrict__ phi_prev;
@@ -377,9 +444,10 @@ template <typename T, int DIM> struct CollideArgs {
   int phiu_valid;
 };
 
-template <typename T, int DIM, int BS>
-__global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, DIM> a, Phys<T> ph) {
+template <typename T, typename S, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, S, DIM> a, Phys<T> ph) {
   constexpr int Q = Lat<DIM>::Q;
+  constexpr bool DEV = !std::is_same<T, S>::value;
   const Grid& g = a.g;
   const long long N = g.N;
   const uint32_t x = blockIdx.x * BS + threadIdx.x;
@@ -391,8 +459,8 @@ __global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, DIM> a, Phys<T> p
   const T eps = (m & M_PARTIAL) ? a.eps[x] : T(0);
 
   T f[Q], gg[Q];
-  pull<T, DIM>(a.Fin, N, x, c, g, m, eps, a.local != 0, f);
-  pull<T, DIM>(a.Gin, N, x, c, g, m, eps, a.local != 0, gg);
+  pull<T, S, DIM, DEV>(a.Fin, N, x, c, g, m, eps, a.local != 0, a.phi_prev, ph, f);
+  pull<T, S, DIM, false>(a.Gin, N, x, c, g, m, eps, a.local != 0, a.phi_prev, ph, gg);
   T pprev = T(0);
   if (!a.local) {
     if ((m & M_TOP) || a.phiu_valid) pprev = a.phi_prev[x];
@@ -457,7 +525,7 @@ __global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, DIM> a, Phys<T> p
       }
     }
   }
-  auto S = [&](int aa, int bb) -> T {
+  auto SIG = [&](int aa, int bb) -> T {
     if (aa > bb) { int t = aa; aa = bb; bb = t; }
     // index of (aa,bb) in the packed upper triangle
     int k = 0;
@@ -471,9 +539,9 @@ __global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, DIM> a, Phys<T> p
   for (int d = 0; d < DIM; ++d) {
     const T Fs = mu * gr[d];
     const T Fp = (((-P) * ph.drho) * ph.cs2) * gr[d];
-    T sg = S(d, 0) * gr[0];
+    T sg = SIG(d, 0) * gr[0];
 #pragma unroll
-    for (int b = 1; b < DIM; ++b) sg = sg + S(d, b) * gr[b];
+    for (int b = 1; b < DIM; ++b) sg = sg + SIG(d, b) * gr[b];
     const T Fm = (ph.visc * sg) * ph.drho;
     const T Fb = ph.body_on ? ph.gforce[d] * rho : T(0);
     T t;
@@ -543,8 +611,14 @@ __global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, DIM> a, Phys<T> p
     const T fs = (f[i] - ph.om_f * (f[i] - fe)) + fsrc;
     const T G = ((ns * ph.w[i]) * edot<DIM>(i, F)) / rc;
     const T gs = (gg[i] - omg * (gg[i] - ge)) + G;
-    a.Fout[(long long)i * N + x] = fs;
-    a.Gout[(long long)i * N + x] = gs;
+    T fst = fs;
+    if constexpr (DEV) {
+      const int off = nb_offset<DIM>(g, c, E<DIM>(i, 0), E<DIM>(i, 1),
+                                     DIM == 3 ? E<DIM>(i, 2) : 0);
+      fst = fs - ph.w[i] * a.phi_cur[(long long)((int)x + off)];
+    }
+    Stor<S>::st(a.Fout, (long long)i * N + x, fst);
+    Stor<S>::st(a.Gout, (long long)i * N + x, gs);
   }
 #pragma unroll
   for (int d = 0; d < DIM; ++d) a.u[(long long)d * N + x] = un[d];
@@ -557,9 +631,10 @@ enum ObsKind : int {
   OBS_PHI = 0, OBS_RHO = 1, OBS_P = 3, OBS_MU = 4, OBS_F = 5, OBS_G = 6,
 };
 
-template <typename T, int DIM> struct ObsArgs {
+template <typename T, typename S, int DIM> struct ObsArgs {
   Grid g;
-  const T* __restrict__ Fsrc;   // F/G source buffer
+  const S* __restrict__ Fsrc;   // F/G source buffer
+  const T* __restrict__ pref;   // phi reference of 16-bit f slots
   const T* __restrict__ phi_cur;
   const T* __restrict__ phi_prev;
   const uint8_t* __restrict__ meta;
@@ -575,8 +650,8 @@ template <typename T, int DIM> struct ObsArgs {
   int init_lap;    // mu with the init_fields Laplacian
 };
 
-template <typename T, int DIM, int BS>
-__global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, DIM> a, Phys<T> ph) {
+template <typename T, typename S, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph) {
   constexpr int Q = Lat<DIM>::Q;
   const Grid& g = a.g;
   const long long N = g.N;
@@ -620,7 +695,11 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, DIM> a, Phys<T> ph) {
     case OBS_G: {
       const T e = (m & M_PARTIAL) ? a.eps[x] : T(0);
       T f[Q];
-      pull<T, DIM>(a.Fsrc, N, x, c, g, m, e, a.local != 0, f);
+      if (a.kind == OBS_F)
+        pull<T, S, DIM, !std::is_same<T, S>::value>(a.Fsrc, N, x, c, g, m, e, a.local != 0,
+                                                    a.pref, ph, f);
+      else
+        pull<T, S, DIM, false>(a.Fsrc, N, x, c, g, m, e, a.local != 0, a.pref, ph, f);
       if (a.kind == OBS_P) {
         a.out[x] = qsum<Q>(f);
         break;
