@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--storage", default=None, choices=[None, "fp32", "bf16", "fp16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run a device copy of this many MiB (a known-size kernel "
+                         "for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the solver is a single-device code "
@@ -94,6 +97,14 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     print(f"[bench] building case on {torch.cuda.get_device_name(0)}", file=sys.stderr,
           flush=True)
+    if args.copy_probe_mb:
+        n_el = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.rand(n_el, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     s, inputs = cfg2(args.storage)
     n = 150 * 150 * 107
     s.step(args.warmup)

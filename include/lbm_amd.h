@@ -171,6 +171,12 @@ int lbm_measure(lbm_solver* s, double* out, void* stream);
  * (3D only).  Synchronises `stream`. */
 int lbm_contact_radius(lbm_solver* s, double cx, double cy, double* out, void* stream);
 
+/* Self-check of the kernels' division by a constant: counts the fp32 inputs
+ * (all 2^32 bit patterns, NaNs excepted) whose quotient by d differs from IEEE
+ * division.  Synchronises `stream`.  No counterpart in the reference (whose
+ * tensor divisions are IEEE). */
+int lbm_check_division(float d, unsigned long long* mismatches, void* stream);
+
 /* Thread-local message for the last failed call. */
 const char* lbm_last_error(void);
 

@@ -117,6 +117,8 @@ template <typename T, int DIM> static Phys<T> make_phys(const lbm_solver* s) {
   const double cs2 = 1.0 / 3.0;
   ph.cs2 = TT<T>(cs2);
   ph.two_cs2 = TT<T>(2.0 * cs2);
+  ph.rcp_cs2 = T(1) / ph.cs2;
+  ph.rcp_two_cs2 = T(1) / ph.two_cs2;
   for (int i = 0; i < Q; ++i) {
     ph.w[i] = (T)w32<DIM>(i);
     const T woc = ph.w[i] / ph.cs2;                 // self._w / self.cs2
@@ -132,6 +134,7 @@ template <typename T, int DIM> static Phys<T> make_phys(const lbm_solver* s) {
   ph.drho = TT<T>(p.rho_l - p.rho_g);
   ph.visc = TT<T>((0.5 - p.tau_g) / p.tau_g);
   ph.xi = TT<T>(p.xi);
+  ph.rcp_xi = T(1) / ph.xi;
   ph.ac_scale = TT<T>(p.ac_scale);
   ph.wet = s->wet;
   ph.mu_wall_add = TT<T>(s->mu_wall * p.mu_wall_scale);
@@ -393,7 +396,8 @@ static int init_impl(lbm_solver* s, const void* phi, const void* u, hipStream_t 
   const Phys<T> ph = make_phys<T, DIM>(s);
   k_init<T, S, DIM, BS><<<s->nblocks, BS, 0, st>>>(s->g, ph, (const T*)phi, (const T*)u,
                                                    (T*)s->phi[0], (T*)s->u, (S*)s->Fb[0],
-                                                   (S*)s->Gb[0], s->partials, s->ctrl);
+                                                   (S*)s->Gb[0], s->partials);
+  k_sum_partials<BS><<<1, BS, 0, st>>>(s->partials, s->nblocks, &s->ctrl->M0);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -428,7 +432,7 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     const int pslot = (int)(s->k & 1), nslot = (int)((s->k + 1) & 1);
     k_phi<T, S, DIM, BS><<<s->nblocks, BS, 0, st>>>(
         s->g, ph, (const S*)s->Fb[src], (const T*)s->phi[pslot], (T*)s->phi[nslot], s->meta,
-        (const T*)s->eps, s->partials, s->ctrl, pslot, nslot, s->local ? 1 : 0);
+        (const T*)s->eps, s->partials, s->ctrl, pslot, s->local ? 1 : 0);
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 1], st));
     CollideArgs<T, S, DIM> a;
     a.g = s->g;
@@ -445,6 +449,10 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     a.eps = (const T*)s->eps;
     a.ampf = (const T*)s->ampf;
     a.ctrl = s->ctrl;
+    a.ctrl_w = s->ctrl;
+    a.partials = s->partials;
+    a.nparts = s->nblocks;
+    a.slot_new = nslot;
     a.slot_prev = pslot;
     a.local = s->local ? 1 : 0;
     a.phiu_valid = s->phiu_valid ? 1 : 0;
@@ -550,6 +558,42 @@ int lbm_profile_steps(lbm_solver* s, int nsteps, void* stream, double* out) {
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   return rc;
+}
+
+// Exhaustive check of divk against IEEE division for one fp32 divisor: every
+// one of the 2^32 fp32 bit patterns x (NaN inputs excepted) must give the
+// bit-identical quotient.
+__global__ void k_check_div(float d, float r, unsigned long long* bad) {
+  const unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned int cnt = 0;
+  for (unsigned int j = 0; j < 64; ++j) {
+    const unsigned int bits = (unsigned int)(t * 64ull + j);
+    const float x = __builtin_bit_cast(float, bits);
+    if (x != x) continue;
+    const float a = divk(x, d, r);
+    const float b = x / d;
+    if (__builtin_bit_cast(unsigned int, a) != __builtin_bit_cast(unsigned int, b)) ++cnt;
+  }
+  if (cnt) atomicAdd(bad, (unsigned long long)cnt);
+}
+
+int lbm_check_division(float d, unsigned long long* mismatches, void* stream) {
+  if (!mismatches) return fail(LBM_EINVAL, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long* dev = nullptr;
+  HIPCHK(hipMalloc(&dev, sizeof(unsigned long long)));
+  hipError_t e = hipMemsetAsync(dev, 0, sizeof(unsigned long long), st);
+  const float r = 1.0f / d;
+  if (e == hipSuccess) {
+    // 2^32 values, 64 per thread, 256 threads per block -> 2^18 blocks
+    k_check_div<<<1u << 18, 256, 0, st>>>(d, r, dev);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(mismatches, dev, sizeof(*mismatches), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(dev);
+  HIPCHK(e);
+  return LBM_OK;
 }
 
 size_t lbm_field_bytes(const lbm_solver* s, int which) {
@@ -708,7 +752,8 @@ static int measure_impl(lbm_solver* s, double* out, hipStream_t st) {
   const Phys<T> ph = make_phys<T, DIM>(s);
   k_measure<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(
       s->g, ph, (const T*)s->phi[s->k & 1], s->meta, s->ctrl, (int)(s->k & 1),
-      s->k > 0 ? 1 : 0, (MeasRec*)s->mparts, &s->ctrl->counter2, s->dout);
+      s->k > 0 ? 1 : 0, (MeasRec*)s->mparts);
+  k_measure_final<DIM, BS><<<1, BS, 0, st>>>((const MeasRec*)s->mparts, s->nblocks, s->dout);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->dout, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -729,7 +774,8 @@ static int contact_impl(lbm_solver* s, double cx, double cy, double* out, hipStr
   const long long ncol = (long long)s->g.nx * s->g.ny;
   k_contact<T, DIM, BS><<<nblk(ncol), BS, 0, st>>>(
       s->g, ph, (const T*)s->phi[s->k & 1], s->meta, s->surf, s->ctrl, (int)(s->k & 1),
-      s->k > 0 ? 1 : 0, cx, cy, s->partials, &s->ctrl->counter2, s->dout + 12);
+      s->k > 0 ? 1 : 0, cx, cy, (double*)s->mparts);
+  k_contact_final<BS><<<1, BS, 0, st>>>((const double*)s->mparts, nblk(ncol), s->dout + 12);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->dout + 12, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

@@ -208,3 +208,82 @@ __global__ __launch_bounds__(BS) void k_surface(Grid g, const uint8_t* __restric
 }
 
 }  // namespace lbm
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------
+// Device-side case generation (geometry/substrate.py:180-267 and
+// lbm/fe_droplet.py:22-112 of the reference), used by the batch engine so a
+// case is built without host round trips.  Same formulas and evaluation order
+// as the numpy generators: fraction = clip(signed distance + 0.5, 0, 1) in
+// fp64 (fp32 for flat), solid = fraction >= 0.5; droplet phi =
+// clip(0.5 + 0.5 tanh(2 (R - r) / xi), 0, 1) evaluated in fp64 and rounded to
+// fp32, u = u_impact * phi.
+// ---------------------------------------------------------------------------
+enum SubstrateKind : int { SUB_FLAT = 0, SUB_RIDGE = 1, SUB_CONVEX = 2, SUB_CONCAVE = 3 };
+
+template <int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_gen_case(Grid g, int kind, double Rg, double* frac64,
+                                                 float* frac32, uint8_t* solid, double cx0,
+                                                 double cy0, double cz0, double R, double xi,
+                                                 float ux, float uy, float uz, float* phi,
+                                                 float* u) {
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  const Coord<DIM> c = decompose<DIM>(g, x);
+  const int k = DIM == 3 ? c.z : c.y;   // last axis: the wall-normal one
+  double fr = k == 0 ? 1.0 : 0.0;
+  const double cx = g.nx / 2.0, cy = g.ny / 2.0;
+  if (kind != SUB_FLAT) {
+    const double dxx = (double)c.x - cx;
+    if (kind == SUB_RIDGE) {
+      const double kk = (double)k;
+      const double dist = sqrt(dxx * dxx + kk * kk);
+      double part = (Rg - dist) + 0.5;
+      part = part < 0.0 ? 0.0 : (part > 1.0 ? 1.0 : part);
+      if (k > 0) fr = part > fr ? part : fr;
+    } else {
+      double s2 = dxx * dxx;
+      if (DIM == 3) {
+        const double dyy = (double)c.y - cy;
+        s2 = s2 + dyy * dyy;
+      }
+      const double dkk = (double)k - Rg;
+      const double dist = sqrt(s2 + dkk * dkk);
+      if (kind == SUB_CONVEX) {
+        double part = (Rg - dist) + 0.5;
+        part = part < 0.0 ? 0.0 : (part > 1.0 ? 1.0 : part);
+        fr = part > fr ? part : fr;
+      } else {
+        double part = (dist - Rg) + 0.5;
+        part = part < 0.0 ? 0.0 : (part > 1.0 ? 1.0 : part);
+        if ((double)k < Rg) fr = part > fr ? part : fr;
+      }
+    }
+  }
+  if (frac64) frac64[x] = fr;
+  if (frac32) frac32[x] = (float)fr;
+  solid[x] = fr >= 0.5 ? 1 : 0;
+  // droplet
+  double r2 = 0.0;
+  {
+    const double a = (double)c.x - cx0;
+    r2 = r2 + a * a;
+    const double b = (double)c.y - cy0;
+    r2 = r2 + b * b;
+    if (DIM == 3) {
+      const double d = (double)c.z - cz0;
+      r2 = r2 + d * d;
+    }
+  }
+  const double r = sqrt(r2);
+  double C = 0.5 + 0.5 * tanh(2.0 * (R - r) / xi);
+  C = C < 0.0 ? 0.0 : (C > 1.0 ? 1.0 : C);
+  const float Cf = (float)C;
+  phi[x] = Cf;
+  u[x] = ux * Cf;
+  u[g.N + x] = uy * Cf;
+  if (DIM == 3) u[2 * g.N + x] = uz * Cf;
+}
+
+}  // namespace lbm

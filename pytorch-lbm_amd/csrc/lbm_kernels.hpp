@@ -42,31 +42,122 @@ struct Ctrl {
 };
 
 // Distribution storage: fp32/fp64 as is; bf16/fp16 with round-to-nearest-even
-// conversion, collided in fp32.
+// conversion, collided in fp32.  Accesses use a per-direction base pointer
+// (uniform, so it lives in SGPRs) plus a 32-bit element index, which maps to
+// the SGPR-base + 32-bit VGPR-offset global addressing mode.
 template <typename S> struct Stor {
-  template <typename T> __device__ static __forceinline__ T ld(const S* p, long long i) {
-    return (T)p[i];
-  }
-  template <typename T> __device__ static __forceinline__ void st(S* p, long long i, T v) {
-    p[i] = (S)v;
-  }
+  template <typename T> __device__ static __forceinline__ T to(S v) { return (T)v; }
+  template <typename T> __device__ static __forceinline__ S from(T v) { return (S)v; }
 };
 template <> struct Stor<__hip_bfloat16> {
-  template <typename T> __device__ static __forceinline__ T ld(const __hip_bfloat16* p, long long i) {
-    return (T)__bfloat162float(p[i]);
+  template <typename T> __device__ static __forceinline__ T to(__hip_bfloat16 v) {
+    return (T)__bfloat162float(v);
   }
-  template <typename T> __device__ static __forceinline__ void st(__hip_bfloat16* p, long long i, T v) {
-    p[i] = __float2bfloat16((float)v);
+  template <typename T> __device__ static __forceinline__ __hip_bfloat16 from(T v) {
+    return __float2bfloat16((float)v);
   }
 };
 template <> struct Stor<__half> {
-  template <typename T> __device__ static __forceinline__ T ld(const __half* p, long long i) {
-    return (T)__half2float(p[i]);
+  template <typename T> __device__ static __forceinline__ T to(__half v) {
+    return (T)__half2float(v);
   }
-  template <typename T> __device__ static __forceinline__ void st(__half* p, long long i, T v) {
-    p[i] = __float2half((float)v);
+  template <typename T> __device__ static __forceinline__ __half from(T v) {
+    return __float2half((float)v);
   }
 };
+
+template <typename E> __device__ __forceinline__ const E& at(const E* base, uint32_t idx) {
+  return *(const E*)((const char*)base + (size_t)(idx * (uint32_t)sizeof(E)));
+}
+template <typename E> __device__ __forceinline__ E& at(E* base, uint32_t idx) {
+  return *(E*)((char*)base + (size_t)(idx * (uint32_t)sizeof(E)));
+}
+
+// Raw buffer resource over one device array: loads/stores take a 32-bit
+// per-lane byte offset and a uniform (SGPR) byte offset, so the per-direction
+// base i*N never costs vector address arithmetic.  Out-of-range offsets read
+// as 0 and drop writes (num_records bound).
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Buf(const void* p, long long bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0,
+                                            bytes >= 0x7fffffffLL ? -1 : (int)bytes,
+                                            0x00020000)) {}
+};
+
+template <typename E> struct BufIO;
+template <> struct BufIO<float> {
+  __device__ static __forceinline__ float ld(const Buf& b, uint32_t vo, uint32_t so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)vo, (int)so, 0));
+  }
+  __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), b.r, (int)vo,
+                                          (int)so, 0);
+  }
+};
+template <> struct BufIO<double> {
+  __device__ static __forceinline__ double ld(const Buf& b, uint32_t vo, uint32_t so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)vo, (int)so, 0));
+  }
+  __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so, double v) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), b.r, (int)vo, (int)so, 0);
+  }
+};
+template <> struct BufIO<__hip_bfloat16> {
+  __device__ static __forceinline__ __hip_bfloat16 ld(const Buf& b, uint32_t vo, uint32_t so) {
+    return __builtin_bit_cast(__hip_bfloat16,
+                              __builtin_amdgcn_raw_buffer_load_b16(b.r, (int)vo, (int)so, 0));
+  }
+  __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so,
+                                            __hip_bfloat16 v) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), b.r, (int)vo,
+                                          (int)so, 0);
+  }
+};
+template <> struct BufIO<__half> {
+  __device__ static __forceinline__ __half ld(const Buf& b, uint32_t vo, uint32_t so) {
+    return __builtin_bit_cast(__half, __builtin_amdgcn_raw_buffer_load_b16(b.r, (int)vo, (int)so, 0));
+  }
+  __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so, __half v) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), b.r, (int)vo,
+                                          (int)so, 0);
+  }
+};
+
+// Element idx of sub-array i (each of N elements) of a buffer holding S.
+template <typename T, typename S>
+__device__ __forceinline__ T bget(const Buf& b, uint32_t N, int i, uint32_t idx) {
+  return Stor<S>::template to<T>(
+      BufIO<S>::ld(b, idx * (uint32_t)sizeof(S), (uint32_t)i * N * (uint32_t)sizeof(S)));
+}
+template <typename T, typename S>
+__device__ __forceinline__ void bput(const Buf& b, uint32_t N, int i, uint32_t idx, T v) {
+  BufIO<S>::st(b, idx * (uint32_t)sizeof(S), (uint32_t)i * N * (uint32_t)sizeof(S),
+               Stor<S>::template from<T>(v));
+}
+
+template <typename T, typename S>
+__device__ __forceinline__ T ldd(const S* F, long long N, int i, uint32_t idx) {
+  return Stor<S>::template to<T>(at(F + (long long)i * N, idx));
+}
+template <typename T, typename S>
+__device__ __forceinline__ void std_(S* F, long long N, int i, uint32_t idx, T v) {
+  at(F + (long long)i * N, idx) = Stor<S>::template from<T>(v);
+}
+
+// Correctly rounded division by a divisor d whose reciprocal r = RN(1/d) is
+// known: q = RN(x r), residual e = x - q d exactly (one FMA), q' = RN(q + e r)
+// (Markstein's theorem: a faithful q plus one FMA correction with the
+// correctly rounded reciprocal gives RN(x/d), barring over/underflow).  The
+// fp32 case is checked exhaustively against IEEE division for the constant
+// divisors by lbm_check_division; fp64 keeps the IEEE division.
+__device__ __forceinline__ float divk(float x, float d, float r) {
+  const float q = x * r;
+  const float e = __builtin_fmaf(-q, d, x);
+  return __builtin_fmaf(e, r, q);
+}
+__device__ __forceinline__ double divk(double x, double d, double) { return x / d; }
 
 template <typename T> __device__ __forceinline__ T clamp01(T v) {
   return v < T(0) ? T(0) : (v > T(1) ? T(1) : v);
@@ -121,17 +212,16 @@ template <int DIM, typename T> __device__ __forceinline__ T vsq(const T* v) {
 // penalization slots (i, x) read at solid/partial x add w_i phi_ref(x - e_i).
 // ---------------------------------------------------------------------------
 template <typename T, typename S, int DIM, bool DEV>
-__device__ __forceinline__ void pull(const S* __restrict__ F, long long N, uint32_t x,
-                                     const Coord<DIM>& c, const Grid& g, uint8_t m, T eps,
-                                     bool local, const T* __restrict__ pref, const Phys<T>& ph,
+__device__ __forceinline__ void pull(const Buf& F, uint32_t N, uint32_t x, const Coord<DIM>& c,
+                                     const Grid& g, uint8_t m, T eps, bool local,
+                                     const Buf& pref, const Phys<T>& ph,
                                      T (&f)[Lat<DIM>::Q]) {
   constexpr int Q = Lat<DIM>::Q;
-  using L = Stor<S>;
   if (local) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = L::template ld<T>(F, (long long)i * N + x);
+    for (int i = 0; i < Q; ++i) f[i] = bget<T, S>(F, N, i, x);
     if constexpr (DEV) {
-      const T r = pref[x];
+      const T r = bget<T, T>(pref, N, 0, x);
 #pragma unroll
       for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.w[i] * r;
     }
@@ -140,11 +230,11 @@ __device__ __forceinline__ void pull(const S* __restrict__ F, long long N, uint3
   if (m & M_SOLID) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      f[i] = L::template ld<T>(F, (long long)OPP<DIM>(i) * N + x);
+      f[i] = bget<T, S>(F, N, OPP<DIM>(i), x);
       if constexpr (DEV) {
         const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
                                        DIM == 3 ? -E<DIM>(i, 2) : 0);
-        f[i] = f[i] + ph.w[i] * pref[(long long)((int)x + off)];
+        f[i] = f[i] + ph.w[i] * bget<T, T>(pref, N, 0, (uint32_t)((int)x + off));
       }
     }
     return;
@@ -153,10 +243,10 @@ __device__ __forceinline__ void pull(const S* __restrict__ F, long long N, uint3
   for (int i = 0; i < Q; ++i) {
     const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
                                    DIM == 3 ? -E<DIM>(i, 2) : 0);
-    f[i] = L::template ld<T>(F, (long long)i * N + (long long)((int)x + off));
+    f[i] = bget<T, S>(F, N, i, (uint32_t)((int)x + off));
   }
   if constexpr (DEV) {
-    const T r = pref[x];
+    const T r = bget<T, T>(pref, N, 0, x);
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.w[i] * r;
   }
@@ -164,15 +254,22 @@ __device__ __forceinline__ void pull(const S* __restrict__ F, long long N, uint3
     const T a = T(1) - eps;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      T o = L::template ld<T>(F, (long long)OPP<DIM>(i) * N + x);
+      T o = bget<T, S>(F, N, OPP<DIM>(i), x);
       if constexpr (DEV) {
         const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
                                        DIM == 3 ? -E<DIM>(i, 2) : 0);
-        o = o + ph.w[i] * pref[(long long)((int)x + off)];
+        o = o + ph.w[i] * bget<T, T>(pref, N, 0, (uint32_t)((int)x + off));
       }
       f[i] = a * f[i] + eps * o;
     }
   }
+}
+
+template <typename S> __device__ __forceinline__ Buf dist_buf(const S* F, long long N) {
+  return Buf(F, (long long)Lat<3>::Q * N * (long long)sizeof(S));
+}
+template <typename T> __device__ __forceinline__ Buf field_buf(const T* p, long long N, int comps = 1) {
+  return Buf(p, (long long)comps * N * (long long)sizeof(T));
 }
 
 // Mass factor and boundary relax of the previous step applied to a pulled f
@@ -216,51 +313,46 @@ template <int BS> __device__ __forceinline__ double block_sum(double v, double* 
   return r;  // valid in thread 0
 }
 
-// Writes this block's partial; the last block to arrive sums all partials in
-// index order and returns true in thread 0 with *total set.
-template <int BS>
-__device__ __forceinline__ bool grid_sum(double bsum, double* __restrict__ partials,
-                                         unsigned int* counter, double* lds, double* total) {
-  __shared__ int is_last;
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = bsum;
-    __threadfence();
-    const unsigned int t = atomicAdd(counter, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  double v = 0.0;
-  const volatile double* vp = partials;
-  for (unsigned int i = threadIdx.x; i < gridDim.x; i += BS) v += vp[i];
-  const double tot = block_sum<BS>(v, lds);
-  if (threadIdx.x == 0) {
-    *total = tot;
-    *counter = 0u;
-  }
-  return threadIdx.x == 0;
-}
-
 // ---------------------------------------------------------------------------
 // init_fields: phi = clamp(phi_init), u, f = f_eq(phi, u), g = g_eq(0, u),
 // M0 = sum(phi) in fp64 (lbm/fe_ac_solver.py:239-261, 346-367).
 // ---------------------------------------------------------------------------
+// (e_i . v) / cs2 for every direction.  Opposite directions are exact
+// negatives of each other (same terms, negated, summed in the same order, and
+// round-to-nearest is symmetric), so only one of each pair is divided.
 template <typename T, int DIM>
-__device__ __forceinline__ void equilibria(const Phys<T>& ph, int i, T phi, T P, const T* u,
-                                           T usq_t, T* feq, T* geq) {
-  const T ec = edot<DIM>(i, u) / ph.cs2;
+__device__ __forceinline__ void ediv(const Phys<T>& ph, const T* v, T d, T r,
+                                     T (&out)[Lat<DIM>::Q]) {
+  constexpr int Q = Lat<DIM>::Q;
+  out[0] = T(0);
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    if (OPP<DIM>(i) > i) out[i] = divk(edot<DIM>(i, v), d, r);
+  }
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    if (OPP<DIM>(i) < i) out[i] = -out[OPP<DIM>(i)];
+  }
+}
+
+// f_eq and g_eq (lbm/fe_ac_solver.py:346-367) from ec = (e.u)/cs2 and
+// usq_t = |u|^2 / (2 cs2).
+template <typename T>
+__device__ __forceinline__ T feq_of(const Phys<T>& ph, int i, T phi, T ec, T usq_t) {
   const T sq = (ec * ec) / T(2);
-  if (feq) *feq = phi * (ph.w[i] * (((T(1) + ec) + sq) - usq_t));
-  if (geq) *geq = P * ph.w[i] + ph.w[i] * ((ec + sq) - usq_t);
+  return phi * (ph.w[i] * (((T(1) + ec) + sq) - usq_t));
+}
+template <typename T>
+__device__ __forceinline__ T geq_of(const Phys<T>& ph, int i, T P, T ec, T usq_t) {
+  const T sq = (ec * ec) / T(2);
+  return P * ph.w[i] + ph.w[i] * ((ec + sq) - usq_t);
 }
 
 template <typename T, typename S, int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __restrict__ phi_in,
                                              const T* __restrict__ u_in, T* __restrict__ phi,
                                              T* __restrict__ u, S* __restrict__ F,
-                                             S* __restrict__ G, double* __restrict__ partials,
-                                             Ctrl* ctrl) {
+                                             S* __restrict__ G, double* __restrict__ partials) {
   constexpr bool DEV = !std::is_same<T, S>::value;
   constexpr int Q = Lat<DIM>::Q;
   __shared__ double lds[BS / 64];
@@ -275,60 +367,94 @@ __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __rest
     phi[x] = p;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) u[(long long)d * N + x] = uu[d];
-    const T usq_t = vsq<DIM>(uu) / ph.two_cs2;
+    const T usq_t = divk(vsq<DIM>(uu), ph.two_cs2, ph.rcp_two_cs2);
+    T ec[Q];
+    ediv<T, DIM>(ph, uu, ph.cs2, ph.rcp_cs2, ec);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      T fe, ge;
-      equilibria<T, DIM>(ph, i, p, T(0), uu, usq_t, &fe, &ge);
+      T fe = feq_of<T>(ph, i, p, ec[i], usq_t);
+      const T ge = geq_of<T>(ph, i, T(0), ec[i], usq_t);
       if constexpr (DEV) fe = fe - ph.w[i] * p;
-      Stor<S>::st(F, (long long)i * N + x, fe);
-      Stor<S>::st(G, (long long)i * N + x, ge);
+      std_<T>(F, N, i, x, fe);
+      std_<T>(G, N, i, x, ge);
     }
     v = (double)p;
   }
   const double b = block_sum<BS>(v, lds);
-  double tot;
-  if (grid_sum<BS>(b, partials, &ctrl->counter, lds, &tot)) ctrl->M0 = tot;
+  if (threadIdx.x == 0) partials[blockIdx.x] = b;
+}
+
+// One block: fixed-order sum of n partials -> *out (M0 after k_init).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sum_partials(const double* __restrict__ partials,
+                                                     unsigned int n, double* out) {
+  __shared__ double lds[BS / 64];
+  double v = 0.0;
+  for (unsigned int i = threadIdx.x; i < n; i += BS) v += partials[i];
+  const double t = block_sum<BS>(v, lds);
+  if (threadIdx.x == 0) *out = t;
 }
 
 // ---------------------------------------------------------------------------
 // Pass A: start-of-step phi and its fp64 sum -> mass factor of this step.
 // ---------------------------------------------------------------------------
+template <typename T, typename S, int DIM>
+__device__ __forceinline__ T phi_node(const Grid& g, const Phys<T>& ph, const S* __restrict__ Fin,
+                                      const T* __restrict__ phi_prev,
+                                      const uint8_t* __restrict__ meta,
+                                      const T* __restrict__ eps, T cprev, bool local,
+                                      uint32_t x) {
+  constexpr int Q = Lat<DIM>::Q;
+  const long long N = g.N;
+  const uint8_t m = meta[x];
+  const Coord<DIM> c = decompose<DIM>(g, x);
+  const T e = (m & M_PARTIAL) ? eps[x] : T(0);
+  T f[Q];
+  const Buf bF = dist_buf(Fin, N), bP = field_buf(phi_prev, N);
+  pull<T, S, DIM, !std::is_same<T, S>::value>(bF, (uint32_t)N, x, c, g, m, e, local, bP, ph, f);
+  if (!local) scale_f<T, DIM>(ph, f, cprev, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr);
+  T p = clamp01(qsum<Q>(f));
+  if (m & M_SOLID) p = T(0);
+  return p;
+}
+
 template <typename T, typename S, int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_phi(Grid g, Phys<T> ph, const S* __restrict__ Fin,
                                             const T* __restrict__ phi_prev,
                                             T* __restrict__ phi_out,
                                             const uint8_t* __restrict__ meta,
                                             const T* __restrict__ eps,
-                                            double* __restrict__ partials, Ctrl* ctrl,
-                                            int slot_prev, int slot_new, int local) {
-  constexpr int Q = Lat<DIM>::Q;
+                                            double* __restrict__ partials, const Ctrl* ctrl,
+                                            int slot_prev, int local) {
   __shared__ double lds[BS / 64];
   const long long N = g.N;
   const uint32_t x = blockIdx.x * BS + threadIdx.x;
   double v = 0.0;
   if (x < N) {
-    const uint8_t m = meta[x];
-    const Coord<DIM> c = decompose<DIM>(g, x);
-    const T e = (m & M_PARTIAL) ? eps[x] : T(0);
-    T f[Q];
-    pull<T, S, DIM, !std::is_same<T, S>::value>(Fin, N, x, c, g, m, e, local != 0, phi_prev,
-                                                ph, f);
-    if (!local) {
-      const T cf = (T)ctrl->c[slot_prev];
-      scale_f<T, DIM>(ph, f, cf, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr);
-    }
-    T p = clamp01(qsum<Q>(f));
-    if (m & M_SOLID) p = T(0);
-    phi_out[x] = p;
+    const T cf = local ? T(1) : (T)ctrl->c[slot_prev];
+    const T p = phi_node<T, S, DIM>(g, ph, Fin, phi_prev, meta, eps, cf, local != 0, x);
+    at(phi_out, x) = p;
     v = (double)p;
   }
+  // block partial of sum(phi); pass B's first block adds them up (after this
+  // kernel has finished, so no inter-block fence is needed here)
   const double b = block_sum<BS>(v, lds);
-  double m;
-  if (grid_sum<BS>(b, partials, &ctrl->counter, lds, &m)) {
-    ctrl->mass[slot_new] = m;
+  if (threadIdx.x == 0) partials[blockIdx.x] = b;
+}
+
+// Mass factor of a step from pass A's partials (fixed order): called by every
+// thread of one block of pass B.
+template <int BS>
+__device__ __forceinline__ void finalize_mass(const double* __restrict__ partials,
+                                              unsigned int n, Ctrl* ctrl, int slot) {
+  __shared__ double lds[BS / 64];
+  double v = 0.0;
+  for (unsigned int i = threadIdx.x; i < n; i += BS) v += partials[i];
+  const double m = block_sum<BS>(v, lds);
+  if (threadIdx.x == 0) {
+    ctrl->mass[slot] = m;
     const double M0 = ctrl->M0;
-    ctrl->c[slot_new] = m > 1e-6 ? M0 / (m > 1e-6 ? m : 1e-6) : 1.0;
+    ctrl->c[slot] = m > 1e-6 ? M0 / m : 1.0;
   }
 }
 
@@ -343,6 +469,7 @@ __device__ __forceinline__ void stencil(const Phys<T>& ph, const Grid& g,
                                         const Coord<DIM>& c, T p, uint32_t wm, const float* n32,
                                         T* grad, T* lap, bool init_lap) {
   constexpr int Q = Lat<DIM>::Q;
+  const Buf bphi = field_buf(phi, g.N);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) grad[d] = T(0);
   T l = T(0);
@@ -350,7 +477,7 @@ __device__ __forceinline__ void stencil(const Phys<T>& ph, const Grid& g,
   for (int i = 1; i < Q; ++i) {
     const int off = nb_offset<DIM>(g, c, E<DIM>(i, 0), E<DIM>(i, 1),
                                    DIM == 3 ? E<DIM>(i, 2) : 0);
-    T psi = phi[(long long)((int)x + off)];
+    T psi = bget<T, T>(bphi, (uint32_t)g.N, 0, (uint32_t)((int)x + off));
     if (wm & (1u << i)) {
       // n . e_i accumulated in fp32 from zero (lbm/fe_wetting.py:237-239)
       float nde = 0.0f;
@@ -439,19 +566,22 @@ template <typename T, typename S, int DIM> struct CollideArgs {
   const T* __restrict__ eps;
   const T* __restrict__ ampf;
   const Ctrl* ctrl;
+  Ctrl* ctrl_w;              // same block; written only by block 0 (slot_new)
+  const double* partials;    // pass A block partials of sum(phi)
+  unsigned int nparts;
   int slot_prev;
+  int slot_new;
   int local;
   int phiu_valid;
 };
 
-template <typename T, typename S, int DIM, int BS>
-__global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, S, DIM> a, Phys<T> ph) {
+template <typename T, typename S, int DIM>
+__device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
+                                             const Phys<T>& ph, uint32_t x) {
   constexpr int Q = Lat<DIM>::Q;
   constexpr bool DEV = !std::is_same<T, S>::value;
   const Grid& g = a.g;
   const long long N = g.N;
-  const uint32_t x = blockIdx.x * BS + threadIdx.x;
-  if (x >= N) return;
   const uint8_t m = a.meta[x];
   const bool solid = m & M_SOLID;
   const bool bnd = m & M_BOUNDARY;
@@ -459,8 +589,12 @@ __global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, S, DIM> a, Phys<T
   const T eps = (m & M_PARTIAL) ? a.eps[x] : T(0);
 
   T f[Q], gg[Q];
-  pull<T, S, DIM, DEV>(a.Fin, N, x, c, g, m, eps, a.local != 0, a.phi_prev, ph, f);
-  pull<T, S, DIM, false>(a.Gin, N, x, c, g, m, eps, a.local != 0, a.phi_prev, ph, gg);
+  const uint32_t N32 = (uint32_t)N;
+  const Buf bFi = dist_buf(a.Fin, N), bGi = dist_buf(a.Gin, N);
+  const Buf bPp = field_buf(a.phi_prev, N), bPc = field_buf(a.phi_cur, N);
+  const Buf bU = field_buf(a.u, N, DIM);
+  pull<T, S, DIM, DEV>(bFi, N32, x, c, g, m, eps, a.local != 0, bPp, ph, f);
+  pull<T, S, DIM, false>(bGi, N32, x, c, g, m, eps, a.local != 0, bPp, ph, gg);
   T pprev = T(0);
   if (!a.local) {
     if ((m & M_TOP) || a.phiu_valid) pprev = a.phi_prev[x];
@@ -470,7 +604,7 @@ __global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, S, DIM> a, Phys<T
   }
 
   // S1/S2: phi, rho, stencil, mu, correction
-  const T p = a.phi_cur[x];
+  const T p = at(a.phi_cur, x);
   const T rho = p * ph.rho_l + (T(1) - p) * ph.rho_g;
   const T rho_eff = rho < ph.rho_floor ? ph.rho_floor : rho;
   float n32[DIM];
@@ -495,21 +629,21 @@ __global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, S, DIM> a, Phys<T
   const T P = qsum<Q>(gg);
   T uo[DIM];
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) uo[d] = a.u[(long long)d * N + x];
+  for (int d = 0; d < DIM; ++d) uo[d] = bget<T, T>(bU, N32, d, x);
   constexpr int NS = DIM == 3 ? 6 : 3;
   T sig[NS];
 #pragma unroll
   for (int k = 0; k < NS; ++k) sig[k] = T(0);
   {
-    const T usq_t = vsq<DIM>(uo) / ph.two_cs2;
+    const T usq_t = divk(vsq<DIM>(uo), ph.two_cs2, ph.rcp_two_cs2);
+    T ec[Q];
+    ediv<T, DIM>(ph, uo, ph.cs2, ph.rcp_cs2, ec);
     bool first[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) first[k] = true;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      T geq;
-      equilibria<T, DIM>(ph, i, T(0), P, uo, usq_t, nullptr, &geq);
-      const T gn = gg[i] - geq;
+      const T gn = gg[i] - geq_of<T>(ph, i, P, ec[i], usq_t);
       int k = 0;
 #pragma unroll
       for (int aa = 0; aa < DIM; ++aa) {
@@ -575,7 +709,7 @@ __global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, S, DIM> a, Phys<T
   {
     T gm = (T)sqrt(vsq<DIM>(gr));
     gm = clamp_lo(gm, T(1e-10));
-    const T lam = ((T(4) * p) * (T(1) - p)) / ph.xi;
+    const T lam = divk((T(4) * p) * (T(1) - p), ph.xi, ph.rcp_xi);
     const T sf = ph.sfac[dist_class(m)];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -601,27 +735,54 @@ __global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, S, DIM> a, Phys<T
     ns = ph.ns_c;
     omg = ph.omg_c;
   }
-  const T usq_n = vsq<DIM>(un) / ph.two_cs2;
+  const Buf bFo = dist_buf(a.Fout, N), bGo = dist_buf(a.Gout, N);
+  const T usq_n = divk(vsq<DIM>(un), ph.two_cs2, ph.rcp_two_cs2);
   const T rc = rho_eff * ph.cs2;
+  const T rrc = T(1) / rc;
+  T ec[Q], fsrc[Q], G[Q];
+  ediv<T, DIM>(ph, un, ph.cs2, ph.rcp_cs2, ec);
+  // source and Guo terms: odd in e, so opposite directions are exact negatives
+  fsrc[0] = T(0);
+  G[0] = T(0);
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    if (OPP<DIM>(i) > i) {
+      fsrc[i] = divk((ph.acoef * ph.w[i]) * edot<DIM>(i, src), ph.cs2, ph.rcp_cs2);
+      G[i] = divk((ns * ph.w[i]) * edot<DIM>(i, F), rc, rrc);
+    }
+  }
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    if (OPP<DIM>(i) < i) {
+      fsrc[i] = -fsrc[OPP<DIM>(i)];
+      G[i] = -G[OPP<DIM>(i)];
+    }
+  }
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    T fe, ge;
-    equilibria<T, DIM>(ph, i, p, P, un, usq_n, &fe, &ge);
-    const T fsrc = ((ph.acoef * ph.w[i]) * edot<DIM>(i, src)) / ph.cs2;
-    const T fs = (f[i] - ph.om_f * (f[i] - fe)) + fsrc;
-    const T G = ((ns * ph.w[i]) * edot<DIM>(i, F)) / rc;
-    const T gs = (gg[i] - omg * (gg[i] - ge)) + G;
+    const T fe = feq_of<T>(ph, i, p, ec[i], usq_n);
+    const T ge = geq_of<T>(ph, i, P, ec[i], usq_n);
+    const T fs = (f[i] - ph.om_f * (f[i] - fe)) + fsrc[i];
+    const T gs = (gg[i] - omg * (gg[i] - ge)) + G[i];
     T fst = fs;
     if constexpr (DEV) {
       const int off = nb_offset<DIM>(g, c, E<DIM>(i, 0), E<DIM>(i, 1),
                                      DIM == 3 ? E<DIM>(i, 2) : 0);
-      fst = fs - ph.w[i] * a.phi_cur[(long long)((int)x + off)];
+      fst = fs - ph.w[i] * bget<T, T>(bPc, N32, 0, (uint32_t)((int)x + off));
     }
-    Stor<S>::st(a.Fout, (long long)i * N + x, fst);
-    Stor<S>::st(a.Gout, (long long)i * N + x, gs);
+    bput<T, S>(bFo, N32, i, x, fst);
+    bput<T, S>(bGo, N32, i, x, gs);
   }
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) a.u[(long long)d * N + x] = un[d];
+  for (int d = 0; d < DIM; ++d) bput<T, T>(bU, N32, d, x, un[d]);
+}
+
+template <typename T, typename S, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, S, DIM> a, Phys<T> ph) {
+  if (blockIdx.x == 0) finalize_mass<BS>(a.partials, a.nparts, a.ctrl_w, a.slot_new);
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= a.g.N) return;
+  collide_node<T, S, DIM>(a, ph, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -695,11 +856,12 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
     case OBS_G: {
       const T e = (m & M_PARTIAL) ? a.eps[x] : T(0);
       T f[Q];
+      const Buf bF = dist_buf(a.Fsrc, N), bR = field_buf(a.pref, N);
       if (a.kind == OBS_F)
-        pull<T, S, DIM, !std::is_same<T, S>::value>(a.Fsrc, N, x, c, g, m, e, a.local != 0,
-                                                    a.pref, ph, f);
+        pull<T, S, DIM, !std::is_same<T, S>::value>(bF, (uint32_t)N, x, c, g, m, e,
+                                                    a.local != 0, bR, ph, f);
       else
-        pull<T, S, DIM, false>(a.Fsrc, N, x, c, g, m, e, a.local != 0, a.pref, ph, f);
+        pull<T, S, DIM, false>(bF, (uint32_t)N, x, c, g, m, e, a.local != 0, bR, ph, f);
       if (a.kind == OBS_P) {
         a.out[x] = qsum<Q>(f);
         break;

@@ -49,18 +49,21 @@ __device__ void combine_rec(MeasRec* s, int n) {
   }
 }
 
+__device__ __forceinline__ void rec_empty(MeasRec& r) {
+  for (int d = 0; d < 3; ++d) { r.mn[d] = 0x7fffffff; r.mx[d] = -1; }
+  r.cnt = 0; r.zsum = 0; r.vol = 0.0; r.cmax = 0.0; r.nan = 0; r.pad = 0;
+}
+
+// Per-block records (block order is the reduction order of k_measure_final).
 template <typename T, int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_measure(Grid g, Phys<T> ph, const T* __restrict__ phi,
                                                 const uint8_t* __restrict__ meta,
                                                 const Ctrl* ctrl, int slot, int scaled,
-                                                MeasRec* __restrict__ parts,
-                                                unsigned int* counter, double* out) {
+                                                MeasRec* __restrict__ parts) {
   __shared__ MeasRec s[BS];
-  __shared__ int is_last;
   const uint32_t x = blockIdx.x * BS + threadIdx.x;
   MeasRec r;
-  for (int d = 0; d < 3; ++d) { r.mn[d] = 0x7fffffff; r.mx[d] = -1; }
-  r.cnt = 0; r.zsum = 0; r.vol = 0.0; r.cmax = 0.0; r.nan = 0; r.pad = 0;
+  rec_empty(r);
   if (x < g.N) {
     const uint8_t m = meta[x];
     const T cf = scaled ? (T)ctrl->c[slot] : T(1);
@@ -80,33 +83,22 @@ __global__ __launch_bounds__(BS) void k_measure(Grid g, Phys<T> ph, const T* __r
   s[threadIdx.x] = r;
   __syncthreads();
   combine_rec<BS>(s, BS);
-  if (threadIdx.x == 0) {
-    parts[blockIdx.x] = s[0];
-    __threadfence();
-    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  // last block: fold partial records in block order, BS at a time
+  if (threadIdx.x == 0) parts[blockIdx.x] = s[0];
+}
+
+// One block folds the n block records in block order and writes
+// out[0..9] = Dx, Dy, Dz, k, volume, C_max, z_com, n_liquid, phi_sum, nan.
+template <int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_measure_final(const MeasRec* __restrict__ parts,
+                                                      unsigned int n, double* out) {
+  __shared__ MeasRec s[BS];
   MeasRec acc;
-  for (int d = 0; d < 3; ++d) { acc.mn[d] = 0x7fffffff; acc.mx[d] = -1; }
-  acc.cnt = 0; acc.zsum = 0; acc.vol = 0.0; acc.cmax = 0.0; acc.nan = 0; acc.pad = 0;
-  for (unsigned int base = 0; base < gridDim.x; base += BS) {
-    __syncthreads();
-    if (base + threadIdx.x < gridDim.x) {
-      const volatile MeasRec* vp = parts + base + threadIdx.x;
-      MeasRec t;
-      for (int d = 0; d < 3; ++d) { t.mn[d] = vp->mn[d]; t.mx[d] = vp->mx[d]; }
-      t.cnt = vp->cnt; t.zsum = vp->zsum; t.vol = vp->vol; t.cmax = vp->cmax;
-      t.nan = vp->nan; t.pad = 0;
-      s[threadIdx.x] = t;
-    } else {
-      MeasRec t;
-      for (int d = 0; d < 3; ++d) { t.mn[d] = 0x7fffffff; t.mx[d] = -1; }
-      t.cnt = 0; t.zsum = 0; t.vol = 0.0; t.cmax = 0.0; t.nan = 0; t.pad = 0;
-      s[threadIdx.x] = t;
-    }
+  rec_empty(acc);
+  for (unsigned int base = 0; base < n; base += BS) {
+    MeasRec t;
+    if (base + threadIdx.x < n) t = parts[base + threadIdx.x];
+    else rec_empty(t);
+    s[threadIdx.x] = t;
     __syncthreads();
     combine_rec<BS>(s, BS);
     if (threadIdx.x == 0) {
@@ -119,6 +111,7 @@ __global__ __launch_bounds__(BS) void k_measure(Grid g, Phys<T> ph, const T* __r
       acc.cmax = b.cmax > acc.cmax ? b.cmax : acc.cmax;
       acc.nan |= b.nan;
     }
+    __syncthreads();
   }
   if (threadIdx.x == 0) {
     double D[3] = {0, 0, 0};
@@ -134,21 +127,18 @@ __global__ __launch_bounds__(BS) void k_measure(Grid g, Phys<T> ph, const T* __r
     out[7] = (double)acc.cnt;
     out[8] = acc.vol;
     out[9] = (double)acc.nan;
-    *counter = 0u;
   }
 }
 
-// Contact-line radius: one thread per column.
+// Contact-line radius: one thread per column; per-block max of r^2.
 template <typename T, int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_contact(Grid g, Phys<T> ph, const T* __restrict__ phi,
                                                 const uint8_t* __restrict__ meta,
                                                 const int16_t* __restrict__ surf,
                                                 const Ctrl* ctrl, int slot, int scaled,
                                                 double cx, double cy,
-                                                double* __restrict__ parts,
-                                                unsigned int* counter, double* out) {
+                                                double* __restrict__ parts) {
   __shared__ double s[BS];
-  __shared__ int is_last;
   const long long col = (long long)blockIdx.x * BS + threadIdx.x;
   const long long ncol = (long long)g.nx * g.ny;
   double r2 = -1.0;
@@ -172,21 +162,16 @@ __global__ __launch_bounds__(BS) void k_contact(Grid g, Phys<T> ph, const T* __r
     if ((int)threadIdx.x < w && s[threadIdx.x + w] > s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    parts[blockIdx.x] = s[0];
-    __threadfence();
-    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  if (threadIdx.x == 0) {
-    double best = -1.0;
-    const volatile double* vp = parts;
-    for (unsigned int i = 0; i < gridDim.x; ++i) best = vp[i] > best ? vp[i] : best;
-    *out = best < 0 ? 0.0 : sqrt(best);
-    *counter = 0u;
-  }
+  if (threadIdx.x == 0) parts[blockIdx.x] = s[0];
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_contact_final(const double* __restrict__ parts,
+                                                      unsigned int n, double* out) {
+  if (threadIdx.x != 0) return;
+  double best = -1.0;
+  for (unsigned int i = 0; i < n; ++i) best = parts[i] > best ? parts[i] : best;
+  *out = best < 0 ? 0.0 : sqrt(best);
 }
 
 }  // namespace lbm

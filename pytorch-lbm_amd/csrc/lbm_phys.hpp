@@ -17,6 +17,7 @@ template <typename T> struct Phys {
   T lcoef[19];        // 2 * (w_i / cs2)         (lbm/fe_ac_solver.py:319)
   T linit[19];        // 2 * w_i                 (lbm/fe_ac_solver.py:293)
   T cs2, two_cs2;
+  T rcp_cs2, rcp_two_cs2, rcp_xi;   // RN(1/cs2), RN(1/(2 cs2)), RN(1/xi)
   T four_beta, kappa, rho_l, rho_g, rho_floor, drho, visc, xi, ac_scale;
   T mu_wall_add;      // linear mode: mu_wall * mu_wall_scale
   T phi_c;            // quadratic mode

@@ -73,6 +73,8 @@ _SIGS = {
                                           ctypes.POINTER(ctypes.c_double), _vp]),
     "lbm_last_error": (ctypes.c_char_p, []),
     "lbm_graph_capture": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbm_check_division": (ctypes.c_int, [ctypes.c_float, ctypes.POINTER(ctypes.c_ulonglong),
+                                          _vp]),
     "lbm_graph_launch": (ctypes.c_int, [_vp, _vp]),
     "lbm_profile_steps": (ctypes.c_int, [_vp, ctypes.c_int, _vp,
                                          ctypes.POINTER(ctypes.c_double)]),

@@ -157,8 +157,10 @@ def test_fp32_vs_fp64_same_kernels(name):
 def test_mass_and_determinism():
     need_gpu()
     print("[gpu] mass/determinism", flush=True)
-    a, z, m = gpu_solver("ridge162_readme_f32", torch.float32)
-    b, _, _ = gpu_solver("ridge162_readme_f32", torch.float32)
+    # production construction (boundary_relax = 0): the mass factor keeps
+    # sum(phi) at M0 up to the clamp at 1 (scripts/validate_solver.py:29 gate)
+    a, z, m = gpu_solver("ridge162_prod_f32", torch.float32)
+    b, _, _ = gpu_solver("ridge162_prod_f32", torch.float32)
     for _ in range(30):
         a.step()
         b.step()
@@ -187,3 +189,18 @@ def test_graph_replay_matches_eager():
         for key in ("phi", "u", "f", "P"):
             np.testing.assert_array_equal(np_(getattr(e, key)), np_(getattr(g, key)))
     assert e.step_count == g.step_count == 13
+
+
+def test_division_by_constants_is_ieee_exact():
+    """The kernels divide by cs2, 2 cs2 and xi with a precomputed reciprocal plus
+    one FMA correction; over all 2^32 fp32 inputs this equals IEEE division."""
+    need_gpu()
+    import ctypes
+    from lbm import _native as nat
+    for d in (np.float32(1.0 / 3.0), np.float32(2.0 / 3.0), np.float32(4.0), np.float32(5.0),
+              np.float32(0.5 * 1.0 / 3.0)):
+        bad = ctypes.c_ulonglong(0)
+        nat.check(nat.lib.lbm_check_division(ctypes.c_float(float(d)), ctypes.byref(bad),
+                                             nat.stream_ptr()), "lbm_check_division")
+        print(f"[gpu] division by {float(d)!r}: {bad.value} mismatches", flush=True)
+        assert bad.value == 0
