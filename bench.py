@@ -35,6 +35,10 @@ B_STEP = 356
 # The collide kernel alone: f, g read+write (304), u read+write (24), phi read
 # (4), previous phi read (4; phi*u_prev is recomputed from it) -> 336 B/LU.
 B_COLLIDE = 336
+# 16-bit distribution storage: f and g at 2 B -> 152 + 24 + 24 + 4 = 204 B/LU
+# for the step, 152 + 24 + 4 + 4 = 184 B/LU for the collide kernel.
+B_STEP_16 = 204
+B_COLLIDE_16 = 184
 
 
 def cfg2(storage=None, dtype="float32"):
@@ -84,6 +88,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--storage", default=None, choices=[None, "fp32", "bf16", "fp16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-autotune", action="store_true", help="keep 256-thread blocks")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a device copy of this many MiB (a known-size kernel "
@@ -107,6 +112,12 @@ def main():
         del src, dst
     s, inputs = cfg2(args.storage)
     n = 150 * 150 * 107
+    sixteen = args.storage in ("bf16", "fp16")
+    b_step = B_STEP_16 if sixteen else B_STEP
+    b_coll = B_COLLIDE_16 if sixteen else B_COLLIDE
+    bs = (256, 256)
+    if not args.no_autotune:
+        bs = s.autotune(5)          # part of the warmup: 18 real steps
     s.step(args.warmup)
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -123,8 +134,8 @@ def main():
     # per-kernel durations with events on the launch stream
     prof_steps = min(100, max(10, args.steps // 5))
     ta, tb, tstep = s.profile_steps(prof_steps)
-    achieved = B_COLLIDE * n / (tb * 1e-3) / 1e9
-    step_gbs = B_STEP * n / (ms_step * 1e-3) / 1e9
+    achieved = b_coll * n / (tb * 1e-3) / 1e9
+    step_gbs = b_step * n / (ms_step * 1e-3) / 1e9
     mass0, mass = s.mass()
     meas = s.measure()
     rec = {
@@ -146,10 +157,12 @@ def main():
                    "geo_amplification": 1.5, "boundary_relax": 0.3,
                    "storage": args.storage or "fp32"},
         "hbm_roofline_pct_step": round(100 * step_gbs / HBM_PEAK_GBS, 1),
+        "bytes_per_lu_step": b_step,
         "roofline": {"bound": "hbm", "kernel": "k_collide (pass B)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_lu": B_COLLIDE, "avg_ms": round(tb, 5)},
+                     "bytes_per_lu": b_coll, "avg_ms": round(tb, 5),
+                     "block_sizes": list(bs)},
         "kernels_ms": {"k_phi": round(ta, 5), "k_collide": round(tb, 5),
                        "step_events": round(tstep, 5)},
         "mass_drift": abs(mass - mass0) / mass0,

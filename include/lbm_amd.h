@@ -137,6 +137,17 @@ int lbm_graph_launch(lbm_solver* s, void* stream);
  * (scripts/run_thesis_sweep.py:173,248). */
 int lbm_profile_steps(lbm_solver* s, int nsteps, void* stream, double* out);
 
+/* Block sizes (threads per workgroup) of pass A and pass B: 128, 256 or 512
+ * (default 256).  Invalidates a captured graph. */
+int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide);
+
+/* Autotuner: times each candidate block size for pass A and pass B over
+ * `nsteps_each` real steps (the solver advances by 3*(nsteps_each+1) steps)
+ * and keeps the fastest of each; reports them in *best_phi / *best_collide.
+ * Synchronises `stream`.  No counterpart in the reference. */
+int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,
+                 int* best_collide);
+
 /* Observable attributes of the solver object (lbm/fe_ac_solver.py:97-105,
  * lbm/fe_wetting.py:140,147,305-306): writes field `which` into dst_dev. */
 int lbm_get_field(lbm_solver* s, int which, void* dst_dev, void* stream);

@@ -78,6 +78,8 @@ struct lbm_solver {
   bool prev_local = true;     // form of buffers at cur^1 (source of P)
   bool phiu_valid = false;
   bool has_frac = false;
+  int bsA = 256, bsB = 256;   // block sizes of pass A / pass B (autotuned)
+  unsigned int nbA = 0, nbB = 0;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int gsteps = 0;
@@ -243,6 +245,7 @@ int lbm_create(const lbm_params* pin, lbm_solver** out) {
   }
   const size_t es = s->esz, Q = s->Q, D = s->dim;
   s->nblocks = nblk(N);
+  s->nbA = s->nbB = s->nblocks;
   hipError_t e = hipSuccess;
 #define ALLOC(ptr, bytes)                                         \
   if (e == hipSuccess) {                                          \
@@ -263,7 +266,7 @@ int lbm_create(const lbm_params* pin, lbm_solver** out) {
   ALLOC(s->wmask, N * 4);
   ALLOC(s->normal, D * N * 4);
   ALLOC(s->surf, (size_t)p.nx * p.ny * 2);
-  ALLOC(s->partials, (size_t)s->nblocks * 8);
+  ALLOC(s->partials, (size_t)((N + 63) / 64 + 1) * 8);   // room for 64-thread pass A blocks
   ALLOC(s->mparts, (size_t)s->nblocks * sizeof(MeasRec));
   ALLOC(s->dout, 16 * 8);
   ALLOC(s->ctrl, sizeof(Ctrl));
@@ -397,7 +400,8 @@ static int init_impl(lbm_solver* s, const void* phi, const void* u, hipStream_t 
   k_init<T, S, DIM, BS><<<s->nblocks, BS, 0, st>>>(s->g, ph, (const T*)phi, (const T*)u,
                                                    (T*)s->phi[0], (T*)s->u, (S*)s->Fb[0],
                                                    (S*)s->Gb[0], s->partials);
-  k_sum_partials<BS><<<1, BS, 0, st>>>(s->partials, s->nblocks, &s->ctrl->M0);
+  k_sum_partials<BS><<<1, BS, 0, st>>>(s->partials, (unsigned int)((s->N + 63) / 64),
+                                       &s->ctrl->M0);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -423,6 +427,14 @@ int lbm_init_fields(lbm_solver* s, const void* phi_dev, const void* u_dev, void*
   return LBM_OK;
 }
 
+// Block sizes the step kernels are instantiated for (the autotuner's space).
+#define LBM_BS_SWITCH(bs, LAUNCH) \
+  switch (bs) {                   \
+    case 128: LAUNCH(128); break; \
+    case 512: LAUNCH(512); break; \
+    default: LAUNCH(256); break;  \
+  }
+
 template <typename T, typename S, int DIM>
 static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev = nullptr) {
   const Phys<T> ph = make_phys<T, DIM>(s);
@@ -430,9 +442,13 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 0], st));
     const int src = s->cur, dst = s->cur ^ 1;
     const int pslot = (int)(s->k & 1), nslot = (int)((s->k + 1) & 1);
-    k_phi<T, S, DIM, BS><<<s->nblocks, BS, 0, st>>>(
-        s->g, ph, (const S*)s->Fb[src], (const T*)s->phi[pslot], (T*)s->phi[nslot], s->meta,
-        (const T*)s->eps, s->partials, s->ctrl, pslot, s->local ? 1 : 0);
+#define LAUNCH_A(B)                                                                       \
+  k_phi<T, S, DIM, B><<<s->nbA, B, 0, st>>>(s->g, ph, (const S*)s->Fb[src],                \
+                                            (const T*)s->phi[pslot], (T*)s->phi[nslot],     \
+                                            s->meta, (const T*)s->eps, s->partials, s->ctrl, \
+                                            pslot, s->local ? 1 : 0)
+    LBM_BS_SWITCH(s->bsA, LAUNCH_A)
+#undef LAUNCH_A
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 1], st));
     CollideArgs<T, S, DIM> a;
     a.g = s->g;
@@ -451,12 +467,14 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     a.ctrl = s->ctrl;
     a.ctrl_w = s->ctrl;
     a.partials = s->partials;
-    a.nparts = s->nblocks;
+    a.nparts = (unsigned int)((s->N + 63) / 64);
     a.slot_new = nslot;
     a.slot_prev = pslot;
     a.local = s->local ? 1 : 0;
     a.phiu_valid = s->phiu_valid ? 1 : 0;
-    k_collide<T, S, DIM, BS><<<s->nblocks, BS, 0, st>>>(a, ph);
+#define LAUNCH_B(B) k_collide<T, S, DIM, B><<<s->nbB, B, 0, st>>>(a, ph)
+    LBM_BS_SWITCH(s->bsB, LAUNCH_B)
+#undef LAUNCH_B
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 2], st));
     s->prev_local = s->local;
     s->local = false;
@@ -593,6 +611,47 @@ int lbm_check_division(float d, unsigned long long* mismatches, void* stream) {
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   (void)hipFree(dev);
   HIPCHK(e);
+  return LBM_OK;
+}
+
+int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  auto ok = [](int b) { return b == 128 || b == 256 || b == 512; };
+  if (!ok(bs_phi) || !ok(bs_collide)) return fail(LBM_EINVAL, "block size must be 128, 256 or 512");
+  s->bsA = bs_phi;
+  s->bsB = bs_collide;
+  s->nbA = (unsigned int)((s->N + bs_phi - 1) / bs_phi);
+  s->nbB = (unsigned int)((s->N + bs_collide - 1) / bs_collide);
+  if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+  if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+  return LBM_OK;
+}
+
+int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,
+                 int* best_collide) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  if (!s->inited) return fail(LBM_ESTATE, "autotune before init_fields()");
+  if (nsteps_each < 2) nsteps_each = 2;
+  const int cand[3] = {128, 256, 512};
+  double bestA = 1e30, bestB = 1e30;
+  int bA = 256, bB = 256;
+  // pass A and pass B are timed independently (events around each launch), so
+  // one sweep over the candidates serves both
+  for (int c = 0; c < 3; ++c) {
+    int rc = lbm_set_block_sizes(s, cand[c], cand[c]);
+    if (rc) return rc;
+    double t[3];
+    rc = lbm_profile_steps(s, 1, stream, t);   // first launch of this shape: untimed
+    if (rc) return rc;
+    rc = lbm_profile_steps(s, nsteps_each, stream, t);
+    if (rc) return rc;
+    if (t[0] < bestA) { bestA = t[0]; bA = cand[c]; }
+    if (t[1] < bestB) { bestB = t[1]; bB = cand[c]; }
+  }
+  int rc = lbm_set_block_sizes(s, bA, bB);
+  if (rc) return rc;
+  if (best_phi) *best_phi = bA;
+  if (best_collide) *best_collide = bB;
   return LBM_OK;
 }
 

@@ -153,6 +153,11 @@ __device__ __forceinline__ void std_(S* F, long long N, int i, uint32_t idx, T v
 // fp32 case is checked exhaustively against IEEE division for the constant
 // divisors by lbm_check_division; fp64 keeps the IEEE division.
 __device__ __forceinline__ float divk(float x, float d, float r) {
+  const float ax = __builtin_fabsf(x);
+  // outside [2^-100, 2^100] (and for 0, inf, NaN) the residual or the product
+  // can under/overflow: use the IEEE division there (rare; exhaustively
+  // checked: the two paths together equal IEEE division for every input)
+  if (__builtin_expect(!(ax >= 0x1p-100f && ax <= 0x1p100f), 0)) return x / d;
   const float q = x * r;
   const float e = __builtin_fmaf(-q, d, x);
   return __builtin_fmaf(e, r, q);
@@ -201,6 +206,20 @@ template <int DIM, typename T> __device__ __forceinline__ T vsq(const T* v) {
 #pragma unroll
   for (int d = 1; d < DIM; ++d) s = s + v[d] * v[d];
   return s;
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8 XCDs
+// (block b runs on XCD b % 8) and every XCD has its own L2, so consecutive
+// blocks -- which share the cache lines of the shifted pulls and the phi
+// stencil -- would land in different L2s.  Remap so that each XCD works on a
+// contiguous range of nodes: XCD j gets logical blocks [start_j, start_j+n_j).
+// ---------------------------------------------------------------------------
+constexpr unsigned int kXCD = 8;
+__device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int nb) {
+  const unsigned int j = b % kXCD, i = b / kXCD;
+  const unsigned int chunk = nb / kXCD, rem = nb % kXCD;
+  return j * chunk + (j < rem ? j : rem) + i;
 }
 
 // ---------------------------------------------------------------------------
@@ -313,6 +332,38 @@ template <int BS> __device__ __forceinline__ double block_sum(double v, double* 
   return r;  // valid in thread 0
 }
 
+// Sum of the 64 nodes of a wave, written to partials[first node / 64]: the
+// reduction tree depends only on node indices, not on block size or order.
+// Waves that lie wholly past the last node write nothing.
+__device__ __forceinline__ void wave_partial(double v, uint32_t x, long long N,
+                                             double* __restrict__ partials) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && (long long)x < N) partials[x >> 6] = v;
+}
+
+// Fixed-order sum of n wave partials by the first 128 threads of a block
+// (valid in thread 0); every thread of the block must call it.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ partials,
+                                               unsigned int n, double* lds128) {
+  if (threadIdx.x < 128) {
+    double a0 = 0.0, a1 = 0.0;
+    unsigned int i = threadIdx.x;
+    for (; i + 128 < n; i += 256) {
+      a0 += partials[i];
+      a1 += partials[i + 128];
+    }
+    if (i < n) a0 += partials[i];
+    lds128[threadIdx.x] = a0 + a1;
+  }
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 128; ++i) r += lds128[i];
+  __syncthreads();
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // init_fields: phi = clamp(phi_init), u, f = f_eq(phi, u), g = g_eq(0, u),
 // M0 = sum(phi) in fp64 (lbm/fe_ac_solver.py:239-261, 346-367).
@@ -355,7 +406,6 @@ __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __rest
                                              S* __restrict__ G, double* __restrict__ partials) {
   constexpr bool DEV = !std::is_same<T, S>::value;
   constexpr int Q = Lat<DIM>::Q;
-  __shared__ double lds[BS / 64];
   const long long N = g.N;
   const uint32_t x = blockIdx.x * BS + threadIdx.x;
   double v = 0.0;
@@ -380,18 +430,15 @@ __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __rest
     }
     v = (double)p;
   }
-  const double b = block_sum<BS>(v, lds);
-  if (threadIdx.x == 0) partials[blockIdx.x] = b;
+  wave_partial(v, x, N, partials);
 }
 
-// One block: fixed-order sum of n partials -> *out (M0 after k_init).
+// One block: fixed-order sum of n wave partials -> *out (M0 after k_init).
 template <int BS>
 __global__ __launch_bounds__(BS) void k_sum_partials(const double* __restrict__ partials,
                                                      unsigned int n, double* out) {
-  __shared__ double lds[BS / 64];
-  double v = 0.0;
-  for (unsigned int i = threadIdx.x; i < n; i += BS) v += partials[i];
-  const double t = block_sum<BS>(v, lds);
+  __shared__ double lds[128];
+  const double t = sum_partials(partials, n, lds);
   if (threadIdx.x == 0) *out = t;
 }
 
@@ -426,9 +473,9 @@ __global__ __launch_bounds__(BS) void k_phi(Grid g, Phys<T> ph, const S* __restr
                                             const T* __restrict__ eps,
                                             double* __restrict__ partials, const Ctrl* ctrl,
                                             int slot_prev, int local) {
-  __shared__ double lds[BS / 64];
   const long long N = g.N;
-  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t x = lb * BS + threadIdx.x;
   double v = 0.0;
   if (x < N) {
     const T cf = local ? T(1) : (T)ctrl->c[slot_prev];
@@ -436,10 +483,9 @@ __global__ __launch_bounds__(BS) void k_phi(Grid g, Phys<T> ph, const S* __restr
     at(phi_out, x) = p;
     v = (double)p;
   }
-  // block partial of sum(phi); pass B's first block adds them up (after this
+  // wave partials of sum(phi); pass B's first block adds them up (after this
   // kernel has finished, so no inter-block fence is needed here)
-  const double b = block_sum<BS>(v, lds);
-  if (threadIdx.x == 0) partials[blockIdx.x] = b;
+  wave_partial(v, x, N, partials);
 }
 
 // Mass factor of a step from pass A's partials (fixed order): called by every
@@ -447,10 +493,8 @@ __global__ __launch_bounds__(BS) void k_phi(Grid g, Phys<T> ph, const S* __restr
 template <int BS>
 __device__ __forceinline__ void finalize_mass(const double* __restrict__ partials,
                                               unsigned int n, Ctrl* ctrl, int slot) {
-  __shared__ double lds[BS / 64];
-  double v = 0.0;
-  for (unsigned int i = threadIdx.x; i < n; i += BS) v += partials[i];
-  const double m = block_sum<BS>(v, lds);
+  __shared__ double lds[128];
+  const double m = sum_partials(partials, n, lds);
   if (threadIdx.x == 0) {
     ctrl->mass[slot] = m;
     const double M0 = ctrl->M0;
@@ -780,7 +824,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
 template <typename T, typename S, int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, S, DIM> a, Phys<T> ph) {
   if (blockIdx.x == 0) finalize_mass<BS>(a.partials, a.nparts, a.ctrl_w, a.slot_new);
-  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  const uint32_t x = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (x >= a.g.N) return;
   collide_node<T, S, DIM>(a, ph, x);
 }

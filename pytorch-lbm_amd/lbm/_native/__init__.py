@@ -76,6 +76,9 @@ _SIGS = {
     "lbm_check_division": (ctypes.c_int, [ctypes.c_float, ctypes.POINTER(ctypes.c_ulonglong),
                                           _vp]),
     "lbm_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "lbm_set_block_sizes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "lbm_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int)]),
     "lbm_profile_steps": (ctypes.c_int, [_vp, ctypes.c_int, _vp,
                                          ctypes.POINTER(ctypes.c_double)]),
 }

@@ -249,6 +249,24 @@ class AllenCahnSolver:
         self._overrides.clear()
         return tuple(float(v) for v in out)
 
+    def autotune(self, nsteps_each=10):
+        """Pick the fastest block sizes for the two step kernels on this case.
+        Advances the solver by 3*(nsteps_each+1) steps."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        with torch.cuda.device(self.device):
+            nat.check(nat.lib.lbm_autotune(self._h, int(nsteps_each), self._stream(),
+                                           ctypes.byref(a), ctypes.byref(b)), "lbm_autotune")
+        n = 3 * (int(nsteps_each) + 1)
+        self.step_count += n
+        self.t += float(n)
+        self._cache.clear()
+        self._overrides.clear()
+        return a.value, b.value
+
+    def set_block_sizes(self, bs_phi, bs_collide):
+        nat.check(nat.lib.lbm_set_block_sizes(self._h, int(bs_phi), int(bs_collide)),
+                  "lbm_set_block_sizes")
+
     def capture_graph(self, nsteps=2):
         """Record `nsteps` (even) steps into a HIP graph owned by the solver; replay
         with run_graph().  Needs one eager step since init_fields."""

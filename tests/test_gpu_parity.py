@@ -192,15 +192,41 @@ def test_graph_replay_matches_eager():
 
 
 def test_division_by_constants_is_ieee_exact():
-    """The kernels divide by cs2, 2 cs2 and xi with a precomputed reciprocal plus
-    one FMA correction; over all 2^32 fp32 inputs this equals IEEE division."""
+    """The kernels divide by cs2, 2 cs2, xi and rho_eff cs2 with a precomputed
+    reciprocal plus one FMA correction (IEEE division outside [2^-100, 2^100]);
+    over all 2^32 fp32 inputs this equals IEEE division."""
     need_gpu()
     import ctypes
     from lbm import _native as nat
+    # cs2, 2 cs2, xi values, and rho_eff * cs2 over its range [0.5/3, 1/3]
     for d in (np.float32(1.0 / 3.0), np.float32(2.0 / 3.0), np.float32(4.0), np.float32(5.0),
-              np.float32(0.5 * 1.0 / 3.0)):
+              np.float32(0.5) * np.float32(1.0 / 3.0), np.float32(0.7312) * np.float32(1.0 / 3.0),
+              np.float32(0.9999) * np.float32(1.0 / 3.0)):
         bad = ctypes.c_ulonglong(0)
         nat.check(nat.lib.lbm_check_division(ctypes.c_float(float(d)), ctypes.byref(bad),
                                              nat.stream_ptr()), "lbm_check_division")
         print(f"[gpu] division by {float(d)!r}: {bad.value} mismatches", flush=True)
         assert bad.value == 0
+
+
+def test_block_sizes_do_not_change_results():
+    """Every block-size choice of the autotuner gives bit-identical states (the
+    mass reduction tree depends on node indices only)."""
+    need_gpu()
+    print("[gpu] block-size invariance", flush=True)
+    ref, z, m = gpu_solver("ridge162_readme_f32", torch.float32)
+    ref.step(12)
+    for bsa, bsb in ((128, 512), (512, 128), (256, 256)):
+        s, _, _ = gpu_solver("ridge162_readme_f32", torch.float32)
+        s.set_block_sizes(bsa, bsb)
+        s.step(12)
+        for key in ("phi", "u", "f", "g"):
+            np.testing.assert_array_equal(np_(getattr(s, key)), np_(getattr(ref, key)))
+
+
+def test_autotune_picks_valid_sizes():
+    need_gpu()
+    s, _, _ = gpu_solver("ridge120_f32", torch.float32)
+    a, b = s.autotune(3)
+    assert a in (128, 256, 512) and b in (128, 256, 512)
+    assert s.step_count == 12
