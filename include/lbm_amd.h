@@ -188,6 +188,70 @@ int lbm_contact_radius(lbm_solver* s, double cx, double cy, double* out, void* s
  * tensor divisions are IEEE). */
 int lbm_check_division(float d, unsigned long long* mismatches, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Batch engine: many cases of uneven grid size advanced together, one launch
+ * per pass per step for all of them (no reference counterpart: the reference
+ * runs sweep cases one after another, scripts/run_thesis_sweep.py:57-284,
+ * scripts/night_queue.py:28-50).  Cases run fp32 compute; precision selects
+ * fp32, bf16 or fp16 distribution storage.
+ * ------------------------------------------------------------------------ */
+typedef struct lbm_batch lbm_batch;
+
+typedef struct lbm_case_spec {
+  lbm_params p;            /* grid, physics and solver options; p.precision must
+                              equal the batch precision */
+  int substrate;           /* 0 flat, 1 ridge, 2 convex, 3 concave
+                              (geometry/substrate.py:162-267) */
+  double R_star, R_d;      /* substrate radius |R_star| * R_d (lattice units) */
+  double center[3];        /* droplet centre (x, y[, z]); a negative last
+                              component places the drop by the reference's
+                              sweep rule (scripts/run_thesis_sweep.py:146-157) */
+  double radius;           /* droplet radius (lbm/fe_droplet.py:22-112) */
+  double u_impact[3];      /* impact velocity */
+  long long steps;         /* step target of the case */
+} lbm_case_spec;
+
+/* New empty batch; hbm_budget_bytes = 0 means unlimited. */
+int lbm_batch_create(int precision, size_t hbm_budget_bytes, lbm_batch** out);
+int lbm_batch_destroy(lbm_batch* b);
+
+/* Device bytes one case of these parameters occupies (admission control). */
+size_t lbm_case_footprint(const lbm_params* p);
+
+/* Build a case on the device (substrate, droplet, geometry precompute,
+ * init_fields) and add it to the batch.  LBM_ENOMEM when it does not fit
+ * the budget.  Does not synchronise other cases' work. */
+int lbm_batch_add(lbm_batch* b, const lbm_case_spec* spec, void* stream, int* case_id);
+
+/* Release a case (synchronises the device). */
+int lbm_batch_remove(lbm_batch* b, int case_id);
+
+/* The solver handle of a case, for lbm_get_field / lbm_measure / ... */
+int lbm_batch_solver(lbm_batch* b, int case_id, lbm_solver** s);
+
+/* Steps done and step target of a case. */
+int lbm_batch_status(lbm_batch* b, int case_id, long long* steps_done, long long* steps_target);
+
+/* Device bytes held by the batch's cases. */
+size_t lbm_batch_bytes_used(const lbm_batch* b);
+
+/* Advance every unfinished case by up to nsteps steps (cases stop at their
+ * target): 3 launches per step for the whole batch, no host synchronisation. */
+int lbm_batch_step(lbm_batch* b, int nsteps, void* stream);
+
+/* Block sizes of the batch kernels (128/256/512). */
+int lbm_batch_set_block_sizes(lbm_batch* b, int bs_phi, int bs_collide);
+
+/* Time the 9 block-size pairs over nsteps_each steps each (the batch advances
+ * by 9*(nsteps_each+1) steps) and keep the fastest.  Synchronises. */
+int lbm_batch_autotune(lbm_batch* b, int nsteps_each, void* stream, int* bs_phi,
+                       int* bs_collide);
+
+/* Capture nsteps (even) batch steps into a graph for the current case set;
+ * every active case must have stepped once and have nsteps left. */
+int lbm_batch_graph_capture(lbm_batch* b, int nsteps, void* stream);
+int lbm_batch_graph_launch(lbm_batch* b, void* stream);
+
 /* Thread-local message for the last failed call. */
 const char* lbm_last_error(void);
 

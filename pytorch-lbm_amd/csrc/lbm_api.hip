@@ -51,7 +51,6 @@ struct lbm_solver {
   double hk = 0.0, mu_wall = 0.0, pck = 0.0, target = 0.0;
   bool has_target = false;
   bool corr_on = false;
-  bool near_wall_any = false;
   void* Fb[2] = {nullptr, nullptr};
   void* Gb[2] = {nullptr, nullptr};
   void* u = nullptr;
@@ -201,7 +200,15 @@ void lbm_params_default(lbm_params* p, int nx, int ny, int nz) {
   p->geometric_wetting = 0; p->geo_amplification = 1.0;
 }
 
+static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipStream_t st);
+
 int lbm_create(const lbm_params* pin, lbm_solver** out) {
+  return create_impl(pin, out, false, nullptr);
+}
+
+// async: zero the buffers on `st` and return without synchronising (used by
+// the batch engine while other cases run).
+static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipStream_t st) {
   if (!pin || !out) return fail(LBM_EINVAL, "null argument");
   const lbm_params& p = *pin;
   if (p.nx < 2 || p.ny < 2 || p.nz < 1)
@@ -247,10 +254,11 @@ int lbm_create(const lbm_params* pin, lbm_solver** out) {
   s->nblocks = nblk(N);
   s->nbA = s->nbB = s->nblocks;
   hipError_t e = hipSuccess;
-#define ALLOC(ptr, bytes)                                         \
-  if (e == hipSuccess) {                                          \
-    e = hipMalloc((void**)&(ptr), (bytes));                       \
-    if (e == hipSuccess) e = hipMemset((ptr), 0, (bytes));        \
+#define ALLOC(ptr, bytes)                                                          \
+  if (e == hipSuccess) {                                                           \
+    e = hipMalloc((void**)&(ptr), (bytes));                                        \
+    if (e == hipSuccess)                                                           \
+      e = async ? hipMemsetAsync((ptr), 0, (bytes), st) : hipMemset((ptr), 0, (bytes)); \
   }
   for (int b = 0; b < 2; ++b) {
     ALLOC(s->Fb[b], Q * N * s->dsz);
@@ -271,7 +279,7 @@ int lbm_create(const lbm_params* pin, lbm_solver** out) {
   ALLOC(s->dout, 16 * 8);
   ALLOC(s->ctrl, sizeof(Ctrl));
 #undef ALLOC
-  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess && !async) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     lbm_destroy(s);
     return fail(e == hipErrorOutOfMemory ? LBM_ENOMEM : LBM_EHIP,
@@ -356,23 +364,10 @@ int lbm_set_geometry(lbm_solver* s, const uint8_t* solid_dev, const void* frac_d
   int rc = DISPATCH(s, CALL);
 #undef CALL
   if (rc) return rc;
-  // near_wall_any: any non-solid node within distance < 10 exists iff there is
-  // at least one boundary node; read back one flag (one-time, synchronous).
-  int any = 0;
-  if (s->wet != WET_NONE) {
-    uint8_t* tmp = nullptr;
-    HIPCHK(hipHostMalloc((void**)&tmp, s->N, hipHostMallocDefault));
-    hipError_t e = hipMemcpyAsync(tmp, s->bnd, s->N, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e == hipSuccess)
-      for (long long i = 0; i < s->N; ++i)
-        if (tmp[i]) { any = 1; break; }
-    (void)hipHostFree(tmp);
-    HIPCHK(e);
-  }
-  s->near_wall_any = any != 0;
+  // The correction acts only at boundary nodes (the only nodes with a nonzero
+  // wall normal), so "no near-wall node" needs no separate flag.
   const double th = s->p.theta_eq * M_PI / 180.0;
-  s->corr_on = s->wet != WET_NONE && s->p.geometric_wetting && s->near_wall_any &&
+  s->corr_on = s->wet != WET_NONE && s->p.geometric_wetting &&
                std::fabs(std::sin(th)) >= 1e-10;
   s->geom = true;
   return LBM_OK;
@@ -848,4 +843,449 @@ int lbm_contact_radius(lbm_solver* s, double cx, double cy, double* out, void* s
 #define CALL(T, D) contact_impl<T, D>(s, cx, cy, out, (hipStream_t)stream)
   return DISPATCH(s, CALL);
 #undef CALL
+}
+
+// ===========================================================================
+// Batch engine: many cases of uneven size advanced together (lbm_batch_*).
+// ===========================================================================
+#include "lbm_batch.hpp"
+
+#include <vector>
+
+struct lbm_batch {
+  int precision = LBM_F32;
+  size_t budget = 0, used = 0;
+  std::vector<lbm_solver*> cs;        // slot -> case (nullptr: free)
+  std::vector<long long> k_end;
+  std::vector<size_t> bytes;
+  int bsA = 256, bsB = 256;
+  bool dirty = true;
+  // device tables
+  void* dcases = nullptr;             // BatchCase<T,S>[cap]
+  size_t dcases_cap = 0;
+  uint32_t* mapA = nullptr;           // [2 * nblocksA]: case, base
+  uint32_t* mapB = nullptr;
+  size_t mapA_cap = 0, mapB_cap = 0;
+  unsigned int nA = 0, nB = 0;        // grid sizes
+  int ncase = 0;                      // descriptor count (== cs.size())
+  // scratch for device-side case generation
+  void* gen = nullptr;
+  size_t gen_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int gsteps = 0;
+};
+
+static size_t case_bytes(const lbm_params& p) {
+  const size_t N = (size_t)p.nx * p.ny * p.nz;
+  const int dim = p.nz == 1 ? 2 : 3, Q = dim == 3 ? 19 : 9;
+  const size_t es = p.precision == LBM_F64 ? 8 : 4;
+  const size_t ds = p.precision == LBM_F64 ? 8 : (p.precision == LBM_F32 ? 4 : 2);
+  const size_t nb = (N + 255) / 256;
+  return 4 * Q * N * ds + 2 * N * es + 2 * N + dim * N * es + N * es + 3 * N + 4 * N +
+         dim * N * 4 + (size_t)p.nx * p.ny * 2 + ((N + 63) / 64 + 1) * 8 +
+         nb * sizeof(MeasRec) + 128 + sizeof(Ctrl);
+}
+
+size_t lbm_case_footprint(const lbm_params* p) { return p ? case_bytes(*p) : 0; }
+
+int lbm_batch_create(int precision, size_t budget, lbm_batch** out) {
+  if (!out) return fail(LBM_EINVAL, "null argument");
+  if (precision < LBM_F32 || precision > LBM_F16) return fail(LBM_EINVAL, "unknown precision");
+  lbm_batch* b = new lbm_batch();
+  b->precision = precision;
+  b->budget = budget;
+  *out = b;
+  return LBM_OK;
+}
+
+static void batch_drop_graph(lbm_batch* b) {
+  if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+}
+
+int lbm_batch_destroy(lbm_batch* b) {
+  if (!b) return LBM_OK;
+  (void)hipDeviceSynchronize();
+  batch_drop_graph(b);
+  for (lbm_solver* s : b->cs)
+    if (s) lbm_destroy(s);
+  void* ptrs[] = {b->dcases, b->mapA, b->mapB, b->gen};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete b;
+  return LBM_OK;
+}
+
+// Device-side case generation into the batch scratch, then the normal
+// set_geometry / init_fields path of the case's solver.
+template <int DIM>
+static int gen_case(lbm_batch* b, lbm_solver* s, const lbm_case_spec& sp, hipStream_t st) {
+  const size_t N = (size_t)s->N;
+  const size_t need = N * (1 + 8 + 4 + 4 + 3 * 4) + 256;
+  if (need > b->gen_cap) {
+    if (b->gen) {
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipFree(b->gen));
+      b->gen = nullptr;
+    }
+    HIPCHK(hipMalloc(&b->gen, need));
+    b->gen_cap = need;
+  }
+  char* base = (char*)b->gen;
+  double* frac64 = (double*)base;
+  float* phi = (float*)(base + 8 * N);
+  float* u = (float*)(base + 12 * N);
+  float* frac32 = (float*)(base + 24 * N);
+  uint8_t* solid = (uint8_t*)(base + 28 * N);
+  const double Rg = std::fabs(sp.R_star) * sp.R_d;
+  const int kind = sp.substrate;
+  k_gen_substrate<DIM, BS><<<s->nblocks, BS, 0, st>>>(
+      s->g, kind, Rg, kind == SUB_FLAT ? nullptr : frac64, kind == SUB_FLAT ? frac32 : nullptr,
+      solid);
+  double c[3] = {sp.center[0], sp.center[1], DIM == 3 ? sp.center[2] : 0.0};
+  const int ax = DIM - 1;   // wall-normal axis
+  if (sp.center[ax] < 0) {
+    // placement rule of the reference's sweep driver
+    // (scripts/run_thesis_sweep.py:146-157): gap 2 above the substrate top at
+    // the centre column (concave: the highest rim), clamped below the lid
+    HIPCHK(hipGetLastError());
+    const long long ncol = DIM == 3 ? (long long)s->g.nx * s->g.ny : s->g.nx;
+    k_surface<DIM, BS><<<nblk(ncol), BS, 0, st>>>(s->g, solid, s->surf);
+    std::vector<int16_t> h((size_t)ncol);
+    HIPCHK(hipMemcpyAsync(h.data(), s->surf, ncol * 2, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int top;
+    if (kind == SUB_CONCAVE) {
+      top = -1;
+      for (int16_t v : h) top = v > top ? v : top;
+    } else {
+      const long long col = DIM == 3 ? (long long)(s->g.nx / 2) * s->g.ny + s->g.ny / 2
+                                     : (long long)(s->g.nx / 2);
+      top = h[(size_t)col];
+    }
+    const int nax = DIM == 3 ? s->g.nz : s->g.ny;
+    double cz = top + 2.0 + sp.radius;
+    const double mx = nax - sp.radius - 2;
+    if (cz > mx) cz = mx;
+    c[ax] = cz;
+  }
+  k_gen_droplet<DIM, BS><<<s->nblocks, BS, 0, st>>>(
+      s->g, c[0], c[1], c[2], sp.radius, s->p.xi, (float)sp.u_impact[0],
+      (float)sp.u_impact[1], (float)sp.u_impact[2], phi, u);
+  HIPCHK(hipGetLastError());
+  int rc = lbm_set_geometry(s, solid, kind == SUB_FLAT ? (const void*)frac32 : (const void*)frac64,
+                            kind == SUB_FLAT ? LBM_FRAC_F32 : LBM_FRAC_F64, st);
+  if (rc) return rc;
+  if (s->esz == 8) {
+    // fp64 runs take fp32-quantised inputs promoted to fp64, as the
+    // reference's generators hand float32 arrays to init_fields
+    return fail(LBM_EINVAL, "batch engine runs fp32 compute (f32, bf16, fp16 storage)");
+  }
+  return lbm_init_fields(s, phi, u, st);
+}
+
+int lbm_batch_add(lbm_batch* b, const lbm_case_spec* sp, void* stream, int* case_id) {
+  if (!b || !sp || !case_id) return fail(LBM_EINVAL, "null argument");
+  if (sp->p.precision != b->precision)
+    return fail(LBM_EINVAL, "case precision differs from the batch precision");
+  if (sp->p.precision == LBM_F64) return fail(LBM_EINVAL, "batch engine runs fp32 compute");
+  if (sp->steps < 0) return fail(LBM_EINVAL, "negative step target");
+  if (sp->substrate < 0 || sp->substrate > 3) return fail(LBM_EINVAL, "bad substrate");
+  const size_t need = case_bytes(sp->p);
+  if (b->budget && b->used + need > b->budget)
+    return fail(LBM_ENOMEM, "case does not fit the batch HBM budget");
+  hipStream_t st = (hipStream_t)stream;
+  lbm_solver* s = nullptr;
+  int rc = create_impl(&sp->p, &s, true, st);
+  if (rc) return rc;
+  rc = s->dim == 3 ? gen_case<3>(b, s, *sp, st) : gen_case<2>(b, s, *sp, st);
+  if (rc) { (void)hipStreamSynchronize(st); lbm_destroy(s); return rc; }
+  int slot = -1;
+  for (size_t i = 0; i < b->cs.size(); ++i)
+    if (!b->cs[i]) { slot = (int)i; break; }
+  if (slot < 0) {
+    slot = (int)b->cs.size();
+    b->cs.push_back(nullptr);
+    b->k_end.push_back(0);
+    b->bytes.push_back(0);
+  }
+  b->cs[slot] = s;
+  b->k_end[slot] = sp->steps;
+  b->bytes[slot] = need;
+  b->used += need;
+  b->dirty = true;
+  *case_id = slot;
+  return LBM_OK;
+}
+
+int lbm_batch_remove(lbm_batch* b, int id) {
+  if (!b || id < 0 || id >= (int)b->cs.size() || !b->cs[id]) return fail(LBM_EINVAL, "bad case id");
+  (void)hipDeviceSynchronize();
+  lbm_destroy(b->cs[id]);
+  b->cs[id] = nullptr;
+  b->used -= b->bytes[id];
+  b->bytes[id] = 0;
+  b->dirty = true;
+  return LBM_OK;
+}
+
+int lbm_batch_solver(lbm_batch* b, int id, lbm_solver** s) {
+  if (!b || !s || id < 0 || id >= (int)b->cs.size() || !b->cs[id]) return fail(LBM_EINVAL, "bad case id");
+  *s = b->cs[id];
+  return LBM_OK;
+}
+
+int lbm_batch_status(lbm_batch* b, int id, long long* done, long long* target) {
+  if (!b || id < 0 || id >= (int)b->cs.size() || !b->cs[id]) return fail(LBM_EINVAL, "bad case id");
+  if (done) *done = b->cs[id]->k;
+  if (target) *target = b->k_end[id];
+  return LBM_OK;
+}
+
+size_t lbm_batch_bytes_used(const lbm_batch* b) { return b ? b->used : 0; }
+
+int lbm_batch_set_block_sizes(lbm_batch* b, int bsA, int bsB) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  auto ok = [](int x) { return x == 128 || x == 256 || x == 512; };
+  if (!ok(bsA) || !ok(bsB)) return fail(LBM_EINVAL, "block size must be 128, 256 or 512");
+  b->bsA = bsA;
+  b->bsB = bsB;
+  b->dirty = true;
+  return LBM_OK;
+}
+
+template <typename T, typename S>
+static int batch_upload(lbm_batch* b, hipStream_t st) {
+  const int n = (int)b->cs.size();
+  std::vector<BatchCase<T, S>> h((size_t)std::max(n, 1));
+  std::vector<uint32_t> mA, mB;
+  for (int c = 0; c < n; ++c) {
+    BatchCase<T, S>& d = h[c];
+    std::memset(&d, 0, sizeof(d));
+    lbm_solver* s = b->cs[c];
+    if (!s) { d.k = 0; d.k_end = 0; continue; }   // free slot: never runs
+    d.g = s->g;
+    d.ph = s->dim == 3 ? make_phys<T, 3>(s) : make_phys<T, 2>(s);
+    for (int i = 0; i < 2; ++i) {
+      d.F[i] = (S*)s->Fb[i];
+      d.G[i] = (S*)s->Gb[i];
+      d.phi[i] = (T*)s->phi[i];
+    }
+    d.u = (T*)s->u;
+    d.eps = (T*)s->eps;
+    d.ampf = (T*)s->ampf;
+    d.meta = s->meta;
+    d.wmask = s->wmask;
+    d.normal = s->normal;
+    d.partials = s->partials;
+    d.ctrl = s->ctrl;
+    d.nbA = (unsigned int)((s->N + b->bsA - 1) / b->bsA);
+    d.nbB = (unsigned int)((s->N + b->bsB - 1) / b->bsB);
+    d.nbA8 = (d.nbA + kXCD - 1) / kXCD * kXCD;
+    d.nbB8 = (d.nbB + kXCD - 1) / kXCD * kXCD;
+    d.k = s->k;
+    d.k_end = b->k_end[c];
+    d.local = s->local ? 1 : 0;
+    d.phiu_valid = s->phiu_valid ? 1 : 0;
+    if (s->k >= b->k_end[c]) continue;   // finished: no blocks
+    const uint32_t baseA = (uint32_t)(mA.size() / 2), baseB = (uint32_t)(mB.size() / 2);
+    for (unsigned int i = 0; i < d.nbA8; ++i) { mA.push_back((uint32_t)c); mA.push_back(baseA); }
+    for (unsigned int i = 0; i < d.nbB8; ++i) { mB.push_back((uint32_t)c); mB.push_back(baseB); }
+  }
+  // split the interleaved (case, base) pairs into two arrays per pass
+  auto upload_map = [&](std::vector<uint32_t>& m, uint32_t** dptr, size_t* cap,
+                        unsigned int* nblocks) -> hipError_t {
+    const size_t nb = m.size() / 2;
+    std::vector<uint32_t> flat(2 * std::max<size_t>(nb, 1));
+    for (size_t i = 0; i < nb; ++i) {
+      flat[i] = m[2 * i];
+      flat[nb + i] = m[2 * i + 1];
+    }
+    *nblocks = (unsigned int)nb;
+    if (flat.size() > *cap) {
+      if (*dptr) { hipError_t e = hipStreamSynchronize(st); if (e) return e; (void)hipFree(*dptr); }
+      hipError_t e = hipMalloc((void**)dptr, flat.size() * 4);
+      if (e) return e;
+      *cap = flat.size();
+    }
+    return hipMemcpyAsync(*dptr, flat.data(), flat.size() * 4, hipMemcpyHostToDevice, st);
+  };
+  HIPCHK(upload_map(mA, &b->mapA, &b->mapA_cap, &b->nA));
+  HIPCHK(upload_map(mB, &b->mapB, &b->mapB_cap, &b->nB));
+  const size_t bytes = h.size() * sizeof(BatchCase<T, S>);
+  if (bytes > b->dcases_cap) {
+    if (b->dcases) { HIPCHK(hipStreamSynchronize(st)); (void)hipFree(b->dcases); }
+    HIPCHK(hipMalloc(&b->dcases, bytes));
+    b->dcases_cap = bytes;
+  }
+  HIPCHK(hipMemcpyAsync(b->dcases, h.data(), bytes, hipMemcpyHostToDevice, st));
+  // pageable source: the copy has been staged when hipMemcpyAsync returns
+  b->ncase = n;
+  b->dirty = false;
+  return LBM_OK;
+}
+
+template <typename T, typename S, int DIM>
+static void batch_launch(lbm_batch* b, hipStream_t st) {
+  const BatchCase<T, S>* dc = (const BatchCase<T, S>*)b->dcases;
+  const unsigned int nA = b->nA, nB = b->nB;
+  BlockMap ma{b->mapA, b->mapA + nA}, mb{b->mapB, b->mapB + nB};
+  if (nA) {
+#define LA(B) k_phi_batch<T, S, DIM, B><<<nA, B, 0, st>>>(dc, ma)
+    LBM_BS_SWITCH(b->bsA, LA)
+#undef LA
+  }
+  if (nB) {
+#define LB(B) k_collide_batch<T, S, DIM, B><<<nB, B, 0, st>>>(dc, mb)
+    LBM_BS_SWITCH(b->bsB, LB)
+#undef LB
+  }
+  k_advance_batch<T, S><<<(b->ncase + 63) / 64, 64, 0, st>>>((BatchCase<T, S>*)b->dcases,
+                                                           b->ncase);
+}
+
+// host mirror of k_advance_batch
+static void batch_host_advance(lbm_batch* b) {
+  for (size_t c = 0; c < b->cs.size(); ++c) {
+    lbm_solver* s = b->cs[c];
+    if (!s || s->k >= b->k_end[c]) continue;
+    s->prev_local = s->local;
+    s->local = false;
+    s->cur ^= 1;
+    s->k += 1;
+    s->phiu_valid = true;
+  }
+}
+
+static int batch_dim(const lbm_batch* b) {
+  for (lbm_solver* s : b->cs)
+    if (s) return s->dim;
+  return 3;
+}
+
+#define BATCH_DISPATCH(b, CALL)                                                        \
+  ((b)->precision == LBM_F32                                                           \
+       ? (batch_dim(b) == 3 ? CALL(float, float, 3) : CALL(float, float, 2))           \
+   : (b)->precision == LBM_BF16                                                        \
+       ? (batch_dim(b) == 3 ? CALL(float, __hip_bfloat16, 3)                           \
+                            : CALL(float, __hip_bfloat16, 2))                          \
+       : (batch_dim(b) == 3 ? CALL(float, __half, 3) : CALL(float, __half, 2)))
+
+template <typename T, typename S, int DIM>
+static int batch_step_impl(lbm_batch* b, int nsteps, hipStream_t st) {
+  for (int it = 0; it < nsteps; ++it) {
+    if (b->dirty) {
+      int rc = batch_upload<T, S>(b, st);
+      if (rc) return rc;
+    }
+    // a case that reaches its target drops out of the block map next step
+    bool finishing = false;
+    for (size_t c = 0; c < b->cs.size(); ++c)
+      if (b->cs[c] && b->cs[c]->k + 1 == b->k_end[c]) finishing = true;
+    batch_launch<T, S, DIM>(b, st);
+    batch_host_advance(b);
+    if (finishing) b->dirty = true;
+  }
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_batch_step(lbm_batch* b, int nsteps, void* stream) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  if (nsteps < 0) return fail(LBM_EINVAL, "negative step count");
+  for (lbm_solver* s : b->cs)
+    if (s && s->dim != batch_dim(b)) return fail(LBM_EINVAL, "a batch holds 2D or 3D cases, not both");
+#define CALL(T, S, D) batch_step_impl<T, S, D>(b, nsteps, (hipStream_t)stream)
+  return BATCH_DISPATCH(b, CALL);
+#undef CALL
+}
+
+int lbm_batch_graph_capture(lbm_batch* b, int nsteps, void* stream) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  if (nsteps < 2 || (nsteps & 1)) return fail(LBM_EINVAL, "nsteps must be even and >= 2");
+  hipStream_t st0 = (hipStream_t)stream;
+  // every active case must have nsteps left and have left its init form, so
+  // each replay is identical
+  for (size_t c = 0; c < b->cs.size(); ++c) {
+    lbm_solver* s = b->cs[c];
+    if (!s || s->k >= b->k_end[c]) continue;
+    if (s->local || b->k_end[c] - s->k < nsteps)
+      return fail(LBM_ESTATE, "graph capture needs one eager step and nsteps left per case");
+  }
+  if (b->dirty) {
+#define CALL(T, S, D) batch_upload<T, S>(b, st0)
+    int rc = BATCH_DISPATCH(b, CALL);
+#undef CALL
+    if (rc) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(st0));
+  batch_drop_graph(b);
+  hipStream_t cs;
+  HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int it = 0; it < nsteps; ++it) {
+#define CALL(T, S, D) (batch_launch<T, S, D>(b, cs), 0)
+      (void)BATCH_DISPATCH(b, CALL);
+#undef CALL
+    }
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    if (e == hipSuccess) {
+      b->graph = g;
+      e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    }
+  }
+  (void)hipStreamDestroy(cs);
+  HIPCHK(e);
+  b->gsteps = nsteps;
+  return LBM_OK;
+}
+
+int lbm_batch_graph_launch(lbm_batch* b, void* stream) {
+  if (!b || !b->gexec) return fail(LBM_ESTATE, "no captured batch graph");
+  if (b->dirty) return fail(LBM_ESTATE, "case set changed since capture");
+  for (size_t c = 0; c < b->cs.size(); ++c) {
+    lbm_solver* s = b->cs[c];
+    if (s && s->k < b->k_end[c] && b->k_end[c] - s->k < b->gsteps)
+      return fail(LBM_ESTATE, "a case has fewer steps left than the captured graph");
+  }
+  HIPCHK(hipGraphLaunch(b->gexec, (hipStream_t)stream));
+  for (int it = 0; it < b->gsteps; ++it) batch_host_advance(b);
+  return LBM_OK;
+}
+
+int lbm_batch_autotune(lbm_batch* b, int nsteps_each, void* stream, int* bsA, int* bsB) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  if (nsteps_each < 1) nsteps_each = 1;
+  hipStream_t st = (hipStream_t)stream;
+  const int cand[3] = {128, 256, 512};
+  float best = 1e30f;
+  int bA = 256, bB = 256;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = LBM_OK;
+  for (int i = 0; i < 3 && rc == LBM_OK; ++i)
+    for (int j = 0; j < 3 && rc == LBM_OK; ++j) {
+      rc = lbm_batch_set_block_sizes(b, cand[i], cand[j]);
+      if (rc) break;
+      rc = lbm_batch_step(b, 1, stream);   // upload + first launch of this shape
+      if (rc) break;
+      (void)hipEventRecord(e0, st);
+      rc = lbm_batch_step(b, nsteps_each, stream);
+      (void)hipEventRecord(e1, st);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) { best = ms; bA = cand[i]; bB = cand[j]; }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  rc = lbm_batch_set_block_sizes(b, bA, bB);
+  if (bsA) *bsA = bA;
+  if (bsB) *bsB = bB;
+  return rc;
 }

@@ -1,0 +1,121 @@
+// Multi-case step kernels: many solver cases of different grid sizes advanced
+// by one launch per pass.  Each case keeps its own device state (the buffers
+// of its lbm_solver) and its own physics constants; a per-block table maps a
+// workgroup to (case, block within the case).  Every case's block range starts
+// at a multiple of the XCD count, so the XCD-aware block order of the
+// single-case kernels carries over unchanged.
+//
+// The per-case step counter lives on the device: pass A and pass B derive the
+// buffer parity from it and skip cases that reached their step target, and a
+// one-block kernel advances it after pass B, so a sequence of batch steps can
+// be captured into a graph and replayed.
+#pragma once
+
+#include "lbm_kernels.hpp"
+
+namespace lbm {
+
+template <typename T, typename S> struct BatchCase {
+  Grid g;
+  Phys<T> ph;
+  S* F[2];
+  S* G[2];
+  T* u;
+  T* phi[2];
+  T* eps;
+  T* ampf;
+  uint8_t* meta;
+  uint32_t* wmask;
+  float* normal;
+  double* partials;
+  Ctrl* ctrl;
+  unsigned int nbA, nbB;       // blocks of this case in pass A / pass B
+  unsigned int nbA8, nbB8;     // rounded up to a multiple of the XCD count
+  long long k, k_end;          // steps done, step target
+  int local, phiu_valid;
+  int pad[2];
+};
+
+// blockIdx -> (case, block within the case) for one pass.
+struct BlockMap {
+  const uint32_t* cs;    // case index per block
+  const uint32_t* base;  // first block of that case
+};
+
+template <typename T, typename S, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_phi_batch(const BatchCase<T, S>* __restrict__ cases,
+                                                  BlockMap map) {
+  const uint32_t ci = map.cs[blockIdx.x];
+  const BatchCase<T, S>& cs = cases[ci];
+  if (cs.k >= cs.k_end) return;
+  const unsigned int i = blockIdx.x - map.base[blockIdx.x];
+  const unsigned int lb = xcd_block(i, cs.nbA8);
+  if (lb >= cs.nbA) return;
+  const long long N = cs.g.N;
+  const uint32_t x = lb * BS + threadIdx.x;
+  const int pslot = (int)(cs.k & 1), nslot = (int)((cs.k + 1) & 1);
+  const int src = pslot;   // buffers alternate with the step count from init
+  double v = 0.0;
+  if (x < N) {
+    const T cf = cs.local ? T(1) : (T)cs.ctrl->c[pslot];
+    const T p = phi_node<T, S, DIM>(cs.g, cs.ph, cs.F[src], cs.phi[pslot], cs.meta, cs.eps, cf,
+                                    cs.local != 0, x);
+    at(cs.phi[nslot], x) = p;
+    v = (double)p;
+  }
+  wave_partial(v, x, N, cs.partials);
+}
+
+template <typename T, typename S, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_collide_batch(const BatchCase<T, S>* __restrict__ cases,
+                                                      BlockMap map) {
+  const uint32_t ci = map.cs[blockIdx.x];
+  const BatchCase<T, S>& cs = cases[ci];
+  if (cs.k >= cs.k_end) return;
+  const unsigned int i = blockIdx.x - map.base[blockIdx.x];
+  const int pslot = (int)(cs.k & 1), nslot = (int)((cs.k + 1) & 1);
+  const long long N = cs.g.N;
+  if (i == 0)
+    finalize_mass<BS>(cs.partials, (unsigned int)((N + 63) / 64), cs.ctrl, nslot);
+  const unsigned int lb = xcd_block(i, cs.nbB8);
+  if (lb >= cs.nbB) return;
+  const uint32_t x = lb * BS + threadIdx.x;
+  if (x >= N) return;
+  CollideArgs<T, S, DIM> a;
+  a.g = cs.g;
+  a.Fin = cs.F[pslot];
+  a.Gin = cs.G[pslot];
+  a.Fout = cs.F[nslot];
+  a.Gout = cs.G[nslot];
+  a.u = cs.u;
+  a.phi_cur = cs.phi[nslot];
+  a.phi_prev = cs.phi[pslot];
+  a.meta = cs.meta;
+  a.wmask = cs.wmask;
+  a.normal = cs.normal;
+  a.eps = cs.eps;
+  a.ampf = cs.ampf;
+  a.ctrl = cs.ctrl;
+  a.ctrl_w = cs.ctrl;
+  a.partials = cs.partials;
+  a.nparts = 0;
+  a.slot_prev = pslot;
+  a.slot_new = nslot;
+  a.local = cs.local;
+  a.phiu_valid = cs.phiu_valid;
+  collide_node<T, S, DIM>(a, cs.ph, x);
+}
+
+// After pass B: every case that stepped advances its counter.
+template <typename T, typename S>
+__global__ void k_advance_batch(BatchCase<T, S>* __restrict__ cases, int n) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  BatchCase<T, S>& cs = cases[c];
+  if (cs.k >= cs.k_end) return;
+  cs.k += 1;
+  cs.local = 0;
+  cs.phiu_valid = 1;
+}
+
+}  // namespace lbm

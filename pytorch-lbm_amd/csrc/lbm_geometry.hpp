@@ -223,11 +223,8 @@ namespace lbm {
 enum SubstrateKind : int { SUB_FLAT = 0, SUB_RIDGE = 1, SUB_CONVEX = 2, SUB_CONCAVE = 3 };
 
 template <int DIM, int BS>
-__global__ __launch_bounds__(BS) void k_gen_case(Grid g, int kind, double Rg, double* frac64,
-                                                 float* frac32, uint8_t* solid, double cx0,
-                                                 double cy0, double cz0, double R, double xi,
-                                                 float ux, float uy, float uz, float* phi,
-                                                 float* u) {
+__global__ __launch_bounds__(BS) void k_gen_substrate(Grid g, int kind, double Rg, double* frac64,
+                                                      float* frac32, uint8_t* solid) {
   const uint32_t x = blockIdx.x * BS + threadIdx.x;
   if (x >= g.N) return;
   const Coord<DIM> c = decompose<DIM>(g, x);
@@ -264,17 +261,24 @@ __global__ __launch_bounds__(BS) void k_gen_case(Grid g, int kind, double Rg, do
   if (frac64) frac64[x] = fr;
   if (frac32) frac32[x] = (float)fr;
   solid[x] = fr >= 0.5 ? 1 : 0;
-  // droplet
+}
+
+template <int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_gen_droplet(Grid g, double cx0, double cy0, double cz0,
+                                                    double R, double xi, float ux, float uy,
+                                                    float uz, float* phi, float* u) {
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  const Coord<DIM> c = decompose<DIM>(g, x);
+  // Python's sum() from integer 0: ((0 + a^2) + b^2) + c^2
   double r2 = 0.0;
-  {
-    const double a = (double)c.x - cx0;
-    r2 = r2 + a * a;
-    const double b = (double)c.y - cy0;
-    r2 = r2 + b * b;
-    if (DIM == 3) {
-      const double d = (double)c.z - cz0;
-      r2 = r2 + d * d;
-    }
+  const double a = (double)c.x - cx0;
+  r2 = r2 + a * a;
+  const double b = (double)c.y - cy0;
+  r2 = r2 + b * b;
+  if (DIM == 3) {
+    const double d = (double)c.z - cz0;
+    r2 = r2 + d * d;
   }
   const double r = sqrt(r2);
   double C = 0.5 + 0.5 * tanh(2.0 * (R - r) / xi);

@@ -51,6 +51,18 @@ class LbmParams(ctypes.Structure):
     ]
 
 
+class LbmCaseSpec(ctypes.Structure):
+    _fields_ = [
+        ("p", LbmParams),
+        ("substrate", ctypes.c_int),
+        ("R_star", ctypes.c_double), ("R_d", ctypes.c_double),
+        ("center", ctypes.c_double * 3),
+        ("radius", ctypes.c_double),
+        ("u_impact", ctypes.c_double * 3),
+        ("steps", ctypes.c_longlong),
+    ]
+
+
 _vp = ctypes.c_void_p
 _SIGS = {
     "lbm_params_default": (None, [ctypes.POINTER(LbmParams), ctypes.c_int, ctypes.c_int,
@@ -77,6 +89,22 @@ _SIGS = {
                                           _vp]),
     "lbm_graph_launch": (ctypes.c_int, [_vp, _vp]),
     "lbm_set_block_sizes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "lbm_case_footprint": (ctypes.c_size_t, [ctypes.POINTER(LbmParams)]),
+    "lbm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "lbm_batch_destroy": (ctypes.c_int, [_vp]),
+    "lbm_batch_add": (ctypes.c_int, [_vp, ctypes.POINTER(LbmCaseSpec), _vp,
+                                     ctypes.POINTER(ctypes.c_int)]),
+    "lbm_batch_remove": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbm_batch_solver": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "lbm_batch_status": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
+                                        ctypes.POINTER(ctypes.c_longlong)]),
+    "lbm_batch_bytes_used": (ctypes.c_size_t, [_vp]),
+    "lbm_batch_step": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
+    "lbm_batch_set_block_sizes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "lbm_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int)]),
+    "lbm_batch_graph_capture": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
+    "lbm_batch_graph_launch": (ctypes.c_int, [_vp, _vp]),
     "lbm_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_int),
                                     ctypes.POINTER(ctypes.c_int)]),
     "lbm_profile_steps": (ctypes.c_int, [_vp, ctypes.c_int, _vp,
