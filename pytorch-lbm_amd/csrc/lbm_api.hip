@@ -220,7 +220,14 @@ static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipS
     return fail(LBM_EINVAL, "tau_l and tau_g must exceed 0.5");
   if (p.stab_mode < 0 || p.stab_mode > 2) return fail(LBM_EINVAL, "bad stab_mode");
   const long long N = (long long)p.nx * p.ny * p.nz;
-  if (N >= (1ll << 31) / 19) return fail(LBM_EINVAL, "grid too large for 32-bit indexing");
+  {
+    // every device array is addressed with 32-bit byte offsets: the largest,
+    // a distribution set, must stay below 4 GiB
+    const long long q = p.nz == 1 ? 9 : 19;
+    const long long ds = p.precision == LBM_F64 ? 8 : (p.precision == LBM_F32 ? 4 : 2);
+    if (q * padded_nodes(N) * ds >= (1ll << 32) || 3 * N * 8 >= (1ll << 32))
+      return fail(LBM_EINVAL, "grid too large: a distribution set must stay below 4 GiB");
+  }
   lbm_solver* s = new lbm_solver();
   s->p = p;
   s->dim = p.nz == 1 ? 2 : 3;
@@ -261,8 +268,8 @@ static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipS
       e = async ? hipMemsetAsync((ptr), 0, (bytes), st) : hipMemset((ptr), 0, (bytes)); \
   }
   for (int b = 0; b < 2; ++b) {
-    ALLOC(s->Fb[b], Q * N * s->dsz);
-    ALLOC(s->Gb[b], Q * N * s->dsz);
+    ALLOC(s->Fb[b], Q * (size_t)padded_nodes(N) * s->dsz);
+    ALLOC(s->Gb[b], Q * (size_t)padded_nodes(N) * s->dsz);
     ALLOC(s->phi[b], N * es);
     ALLOC(s->dist[b], N);
   }
@@ -882,7 +889,7 @@ static size_t case_bytes(const lbm_params& p) {
   const size_t es = p.precision == LBM_F64 ? 8 : 4;
   const size_t ds = p.precision == LBM_F64 ? 8 : (p.precision == LBM_F32 ? 4 : 2);
   const size_t nb = (N + 255) / 256;
-  return 4 * Q * N * ds + 2 * N * es + 2 * N + dim * N * es + N * es + 3 * N + 4 * N +
+  return 4 * Q * (size_t)padded_nodes((long long)N) * ds + 2 * N * es + 2 * N + dim * N * es + N * es + 3 * N + 4 * N +
          dim * N * 4 + (size_t)p.nx * p.ny * 2 + ((N + 63) / 64 + 1) * 8 +
          nb * sizeof(MeasRec) + 128 + sizeof(Ctrl);
 }

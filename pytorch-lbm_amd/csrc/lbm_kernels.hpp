@@ -137,13 +137,26 @@ __device__ __forceinline__ void bput(const Buf& b, uint32_t N, int i, uint32_t i
                Stor<S>::template from<T>(v));
 }
 
-template <typename T, typename S>
-__device__ __forceinline__ T ldd(const S* F, long long N, int i, uint32_t idx) {
-  return Stor<S>::template to<T>(at(F + (long long)i * N, idx));
+// Distribution layout (AoSoA): nodes are grouped in blocks of 64 (one wave of
+// consecutive nodes); a block stores its Q directions one after another, 64
+// values each, so element (i, x) sits at ((x / 64) * Q + i) * 64 + x % 64.  A
+// wave's node-local stores of all directions fill one contiguous region
+// instead of Q regions N elements apart, and pulls read 64-value runs of one
+// direction as before.  Buffers hold padded_nodes(N) nodes.
+__host__ __device__ constexpr long long padded_nodes(long long N) { return (N + 63) & ~63LL; }
+
+template <int Q, typename S> __device__ __forceinline__ uint32_t dvoff(uint32_t idx) {
+  return ((idx >> 6) * (uint32_t)(Q * 64) + (idx & 63u)) * (uint32_t)sizeof(S);
 }
-template <typename T, typename S>
-__device__ __forceinline__ void std_(S* F, long long N, int i, uint32_t idx, T v) {
-  at(F + (long long)i * N, idx) = Stor<S>::template from<T>(v);
+template <int Q, typename T, typename S>
+__device__ __forceinline__ T dget(const Buf& b, int i, uint32_t idx) {
+  return Stor<S>::template to<T>(
+      BufIO<S>::ld(b, dvoff<Q, S>(idx), (uint32_t)i * 64u * (uint32_t)sizeof(S)));
+}
+// store of direction i at the node whose dvoff is vo
+template <int Q, typename T, typename S>
+__device__ __forceinline__ void dput(const Buf& b, int i, uint32_t vo, T v) {
+  BufIO<S>::st(b, vo, (uint32_t)i * 64u * (uint32_t)sizeof(S), Stor<S>::template from<T>(v));
 }
 
 // Correctly rounded division by a divisor d whose reciprocal r = RN(1/d) is
@@ -238,7 +251,7 @@ __device__ __forceinline__ void pull(const Buf& F, uint32_t N, uint32_t x, const
   constexpr int Q = Lat<DIM>::Q;
   if (local) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = bget<T, S>(F, N, i, x);
+    for (int i = 0; i < Q; ++i) f[i] = dget<Q, T, S>(F, i, x);
     if constexpr (DEV) {
       const T r = bget<T, T>(pref, N, 0, x);
 #pragma unroll
@@ -249,7 +262,7 @@ __device__ __forceinline__ void pull(const Buf& F, uint32_t N, uint32_t x, const
   if (m & M_SOLID) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      f[i] = bget<T, S>(F, N, OPP<DIM>(i), x);
+      f[i] = dget<Q, T, S>(F, OPP<DIM>(i), x);
       if constexpr (DEV) {
         const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
                                        DIM == 3 ? -E<DIM>(i, 2) : 0);
@@ -262,7 +275,7 @@ __device__ __forceinline__ void pull(const Buf& F, uint32_t N, uint32_t x, const
   for (int i = 0; i < Q; ++i) {
     const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
                                    DIM == 3 ? -E<DIM>(i, 2) : 0);
-    f[i] = bget<T, S>(F, N, i, (uint32_t)((int)x + off));
+    f[i] = dget<Q, T, S>(F, i, (uint32_t)((int)x + off));
   }
   if constexpr (DEV) {
     const T r = bget<T, T>(pref, N, 0, x);
@@ -273,7 +286,7 @@ __device__ __forceinline__ void pull(const Buf& F, uint32_t N, uint32_t x, const
     const T a = T(1) - eps;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      T o = bget<T, S>(F, N, OPP<DIM>(i), x);
+      T o = dget<Q, T, S>(F, OPP<DIM>(i), x);
       if constexpr (DEV) {
         const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
                                        DIM == 3 ? -E<DIM>(i, 2) : 0);
@@ -284,8 +297,8 @@ __device__ __forceinline__ void pull(const Buf& F, uint32_t N, uint32_t x, const
   }
 }
 
-template <typename S> __device__ __forceinline__ Buf dist_buf(const S* F, long long N) {
-  return Buf(F, (long long)Lat<3>::Q * N * (long long)sizeof(S));
+template <int DIM, typename S> __device__ __forceinline__ Buf dist_buf(const S* F, long long N) {
+  return Buf(F, (long long)Lat<DIM>::Q * padded_nodes(N) * (long long)sizeof(S));
 }
 template <typename T> __device__ __forceinline__ Buf field_buf(const T* p, long long N, int comps = 1) {
   return Buf(p, (long long)comps * N * (long long)sizeof(T));
@@ -420,13 +433,15 @@ __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __rest
     const T usq_t = divk(vsq<DIM>(uu), ph.two_cs2, ph.rcp_two_cs2);
     T ec[Q];
     ediv<T, DIM>(ph, uu, ph.cs2, ph.rcp_cs2, ec);
+    const Buf bF = dist_buf<DIM>(F, N), bG = dist_buf<DIM>(G, N);
+    const uint32_t vx = dvoff<Q, S>(x);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       T fe = feq_of<T>(ph, i, p, ec[i], usq_t);
       const T ge = geq_of<T>(ph, i, T(0), ec[i], usq_t);
       if constexpr (DEV) fe = fe - ph.w[i] * p;
-      std_<T>(F, N, i, x, fe);
-      std_<T>(G, N, i, x, ge);
+      dput<Q, T, S>(bF, i, vx, fe);
+      dput<Q, T, S>(bG, i, vx, ge);
     }
     v = (double)p;
   }
@@ -457,7 +472,7 @@ __device__ __forceinline__ T phi_node(const Grid& g, const Phys<T>& ph, const S*
   const Coord<DIM> c = decompose<DIM>(g, x);
   const T e = (m & M_PARTIAL) ? eps[x] : T(0);
   T f[Q];
-  const Buf bF = dist_buf(Fin, N), bP = field_buf(phi_prev, N);
+  const Buf bF = dist_buf<DIM>(Fin, N), bP = field_buf(phi_prev, N);
   pull<T, S, DIM, !std::is_same<T, S>::value>(bF, (uint32_t)N, x, c, g, m, e, local, bP, ph, f);
   if (!local) scale_f<T, DIM>(ph, f, cprev, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr);
   T p = clamp01(qsum<Q>(f));
@@ -634,7 +649,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
 
   T f[Q], gg[Q];
   const uint32_t N32 = (uint32_t)N;
-  const Buf bFi = dist_buf(a.Fin, N), bGi = dist_buf(a.Gin, N);
+  const Buf bFi = dist_buf<DIM>(a.Fin, N), bGi = dist_buf<DIM>(a.Gin, N);
   const Buf bPp = field_buf(a.phi_prev, N), bPc = field_buf(a.phi_cur, N);
   const Buf bU = field_buf(a.u, N, DIM);
   pull<T, S, DIM, DEV>(bFi, N32, x, c, g, m, eps, a.local != 0, bPp, ph, f);
@@ -779,10 +794,11 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
     ns = ph.ns_c;
     omg = ph.omg_c;
   }
-  const Buf bFo = dist_buf(a.Fout, N), bGo = dist_buf(a.Gout, N);
+  const Buf bFo = dist_buf<DIM>(a.Fout, N), bGo = dist_buf<DIM>(a.Gout, N);
   const T usq_n = divk(vsq<DIM>(un), ph.two_cs2, ph.rcp_two_cs2);
   const T rc = rho_eff * ph.cs2;
   const T rrc = T(1) / rc;
+  const uint32_t vx = dvoff<Q, S>(x);
   T ec[Q], fsrc[Q], G[Q];
   ediv<T, DIM>(ph, un, ph.cs2, ph.rcp_cs2, ec);
   // source and Guo terms: odd in e, so opposite directions are exact negatives
@@ -814,8 +830,8 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
                                      DIM == 3 ? E<DIM>(i, 2) : 0);
       fst = fs - ph.w[i] * bget<T, T>(bPc, N32, 0, (uint32_t)((int)x + off));
     }
-    bput<T, S>(bFo, N32, i, x, fst);
-    bput<T, S>(bGo, N32, i, x, gs);
+    dput<Q, T, S>(bFo, i, vx, fst);
+    dput<Q, T, S>(bGo, i, vx, gs);
   }
 #pragma unroll
   for (int d = 0; d < DIM; ++d) bput<T, T>(bU, N32, d, x, un[d]);
@@ -900,7 +916,7 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
     case OBS_G: {
       const T e = (m & M_PARTIAL) ? a.eps[x] : T(0);
       T f[Q];
-      const Buf bF = dist_buf(a.Fsrc, N), bR = field_buf(a.pref, N);
+      const Buf bF = dist_buf<DIM>(a.Fsrc, N), bR = field_buf(a.pref, N);
       if (a.kind == OBS_F)
         pull<T, S, DIM, !std::is_same<T, S>::value>(bF, (uint32_t)N, x, c, g, m, e,
                                                     a.local != 0, bR, ph, f);

@@ -94,7 +94,7 @@ def test_batch_case_matches_solver_api():
     numpy generators (device tanh may differ from numpy's in the last fp32 bit
     of a few initial values, hence a tolerance)."""
     need_gpu()
-    from geometry.substrate import create_substrate_with_fraction, substrate_surface_height
+    from geometry.substrate import create_substrate_with_fraction
     from lbm.batch import BatchEngine, CaseSpec
     from lbm.fe_ac_solver import AllenCahnSolver
     from lbm.fe_droplet import create_fe_droplet_with_impact
@@ -109,8 +109,10 @@ def test_batch_case_matches_solver_api():
                           geometric_wetting=True)
     solid, frac = create_substrate_with_fraction(24, 24, 20, "ridge", 1.0, 5.0)
     sol.set_solid(solid, frac)
-    h = substrate_surface_height(24, 24, 20, "ridge", 1.0, 5.0)   # placement rule
-    cz = min(int(h[12, 12]) + 2.0 + 5.0, 20 - 5.0 - 2)
+    # placement rule of scripts/run_thesis_sweep.py:146-157 on the solid that
+    # is handed to set_solid (the volume-penalization solid)
+    top = int(np.nonzero(solid[12, 12])[0].max())
+    cz = min(top + 2.0 + 5.0, 20 - 5.0 - 2)
     phi0, _, u0 = create_fe_droplet_with_impact(24, 24, 20, center=(12.0, 12.0, cz), radius=5.0,
                                                 xi=4.0, u_impact=(0, 0, -0.05))
     sol.init_fields(phi0, u0)
