@@ -94,14 +94,22 @@ static void fast_div(uint32_t d, uint32_t* m, uint32_t* sh) {
   *sh = (uint32_t)(p - 32);
 }
 
+// Internal node order: wall-normal (last) axis slowest -- 3D (z, x, y) with y
+// fastest, 2D (y, x) with x fastest.
 static Grid make_grid(int nx, int ny, int nz) {
   Grid g{};
   g.nx = nx; g.ny = ny; g.nz = nz;
-  if (nz > 1) { g.sx = ny * nz; g.sy = nz; }
-  else { g.sx = ny; g.sy = 1; }
+  if (nz > 1) {
+    g.sy = 1; g.sx = ny; g.sz = nx * ny;
+    g.n1 = (uint32_t)ny; g.n2 = (uint32_t)nx;
+  } else {
+    g.sx = 1; g.sy = nx; g.sz = 0;
+    g.n1 = (uint32_t)nx; g.n2 = 1;
+  }
+  g.wx = nx * g.sx; g.wy = ny * g.sy; g.wz = nz * g.sz;
   g.N = (long long)nx * ny * nz;
-  fast_div((uint32_t)nz, &g.mz, &g.sz_shift);
-  fast_div((uint32_t)ny, &g.my, &g.sy_shift);
+  fast_div(g.n1, &g.m1, &g.sh1);
+  fast_div(g.n2, &g.m2, &g.sh2);
   return g;
 }
 
@@ -357,15 +365,11 @@ static int set_geometry_impl(lbm_solver* s, const void* frac, int fdt, hipStream
   (((s)->esz == 8) ? (((s)->dim == 3) ? CALL(double, 3) : CALL(double, 2)) \
                    : (((s)->dim == 3) ? CALL(float, 3) : CALL(float, 2)))
 
-int lbm_set_geometry(lbm_solver* s, const uint8_t* solid_dev, const void* frac_dev,
-                     int frac_dtype, void* stream) {
-  if (!s || !solid_dev) return fail(LBM_EINVAL, "null argument");
-  if (frac_dtype != LBM_FRAC_NONE && !frac_dev) return fail(LBM_EINVAL, "null fraction");
+// Geometry precompute from s->solid and a fraction array, both already in the
+// internal node order.
+static int set_geometry_internal(lbm_solver* s, const void* frac_dev, int frac_dtype,
+                                 hipStream_t st) {
   if (frac_dtype == LBM_FRAC_NONE) frac_dev = nullptr;
-  if (frac_dtype < 0 || frac_dtype > 2) return fail(LBM_EINVAL, "bad fraction dtype");
-  hipStream_t st = (hipStream_t)stream;
-  if (solid_dev != s->solid)
-    HIPCHK(hipMemcpyAsync(s->solid, solid_dev, s->N, hipMemcpyDeviceToDevice, st));
   s->has_frac = frac_dev != nullptr;
 #define CALL(T, D) set_geometry_impl<T, D>(s, frac_dev, frac_dev ? frac_dtype : 0, st)
   int rc = DISPATCH(s, CALL);
@@ -380,6 +384,64 @@ int lbm_set_geometry(lbm_solver* s, const uint8_t* solid_dev, const void* frac_d
   return LBM_OK;
 }
 
+// Public-order array -> internal order (element size es bytes, comps arrays).
+static int to_internal(const lbm_solver* s, const void* pub, void* in, size_t es, int comps,
+                       hipStream_t st) {
+  const Grid g = s->g;
+#define TOI(E)                                                                              \
+  (s->dim == 3 ? (k_to_internal<E, 3, BS><<<s->nblocks, BS, 0, st>>>(g, (const E*)pub, (E*)in, comps), 0) \
+               : (k_to_internal<E, 2, BS><<<s->nblocks, BS, 0, st>>>(g, (const E*)pub, (E*)in, comps), 0))
+  if (es == 1) TOI(uint8_t);
+  else if (es == 2) TOI(uint16_t);
+  else if (es == 4) TOI(uint32_t);
+  else TOI(uint64_t);
+#undef TOI
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+static int to_public(const lbm_solver* s, const void* in, void* pub, size_t es, int comps,
+                     hipStream_t st) {
+  const Grid g = s->g;
+#define TOP(E)                                                                              \
+  (s->dim == 3 ? (k_to_public<E, 3, BS><<<s->nblocks, BS, 0, st>>>(g, (const E*)in, (E*)pub, comps), 0) \
+               : (k_to_public<E, 2, BS><<<s->nblocks, BS, 0, st>>>(g, (const E*)in, (E*)pub, comps), 0))
+  if (es == 1) TOP(uint8_t);
+  else if (es == 2) TOP(uint16_t);
+  else if (es == 4) TOP(uint32_t);
+  else TOP(uint64_t);
+#undef TOP
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_set_geometry(lbm_solver* s, const uint8_t* solid_dev, const void* frac_dev,
+                     int frac_dtype, void* stream) {
+  if (!s || !solid_dev) return fail(LBM_EINVAL, "null argument");
+  if (frac_dtype != LBM_FRAC_NONE && !frac_dev) return fail(LBM_EINVAL, "null fraction");
+  if (frac_dtype < 0 || frac_dtype > 2) return fail(LBM_EINVAL, "bad fraction dtype");
+  if (frac_dtype == LBM_FRAC_NONE) frac_dev = nullptr;
+  hipStream_t st = (hipStream_t)stream;
+  // inputs arrive in the public order: reorder solid into place and the
+  // fraction into a temporary (set-up only)
+  if (solid_dev == s->solid) return fail(LBM_EINVAL, "solid must not alias solver memory");
+  int rc = to_internal(s, solid_dev, s->solid, 1, 1, st);
+  if (rc) return rc;
+  void* tmp = nullptr;
+  if (frac_dev) {
+    const size_t fes = frac_dtype == LBM_FRAC_F64 ? 8 : 4;
+    HIPCHK(hipMalloc(&tmp, (size_t)s->N * fes));
+    rc = to_internal(s, frac_dev, tmp, fes, 1, st);
+  }
+  if (!rc) rc = set_geometry_internal(s, tmp, frac_dtype, st);
+  if (tmp) {
+    const hipError_t e = hipStreamSynchronize(st);
+    (void)hipFree(tmp);
+    if (!rc && e != hipSuccess) rc = fail(LBM_EHIP, "stream synchronize failed");
+  }
+  return rc;
+}
+
 int lbm_set_amplification_field(lbm_solver* s, const void* amp_dev, void* stream) {
   if (!s) return fail(LBM_EINVAL, "null solver");
   hipStream_t st = (hipStream_t)stream;
@@ -392,8 +454,7 @@ int lbm_set_amplification_field(lbm_solver* s, const void* amp_dev, void* stream
     return LBM_OK;
   }
   if (!s->ampf) HIPCHK(hipMalloc(&s->ampf, s->N * s->esz));
-  HIPCHK(hipMemcpyAsync(s->ampf, amp_dev, s->N * s->esz, hipMemcpyDeviceToDevice, st));
-  return LBM_OK;
+  return to_internal(s, amp_dev, s->ampf, s->esz, 1, st);
 }
 
 template <typename T, typename S, int DIM>
@@ -413,7 +474,7 @@ int lbm_init_fields(lbm_solver* s, const void* phi_dev, const void* u_dev, void*
   if (!s->geom) {
     // the reference allows init before set_solid: an all-fluid domain
     HIPCHK(hipMemsetAsync(s->solid, 0, s->N, (hipStream_t)stream));
-    int rc = lbm_set_geometry(s, s->solid, nullptr, LBM_FRAC_NONE, stream);
+    int rc = set_geometry_internal(s, nullptr, LBM_FRAC_NONE, (hipStream_t)stream);
     if (rc) return rc;
   }
 #define CALL(T, S, D) init_impl<T, S, D>(s, phi_dev, u_dev, (hipStream_t)stream)
@@ -677,25 +738,30 @@ size_t lbm_field_bytes(const lbm_solver* s, int which) {
 
 // geometry observables from the packed byte
 template <int BS_>
-__global__ void k_geom_obs(long long N, const uint8_t* __restrict__ meta,
-                           const uint8_t* __restrict__ bnd, int which, void* out) {
+__global__ void k_geom_obs(Grid g, int dim, const uint8_t* __restrict__ meta,
+                           const uint8_t* __restrict__ bnd, int which, void* out_) {
   const long long x = (long long)blockIdx.x * BS_ + threadIdx.x;
-  if (x >= N) return;
+  if (x >= g.N) return;
+  const uint32_t ex = dim == 3 ? ext_index<3>(g, decompose<3>(g, (uint32_t)x))
+                               : ext_index<2>(g, decompose<2>(g, (uint32_t)x));
+  // outputs in the public order
+  uint8_t* out8 = (uint8_t*)out_ + ex;
+  float* outf = (float*)out_ + ex;
   const uint8_t m = meta[x];
   const bool solid = m & M_SOLID;
   const int cls = dist_class(m);
   switch (which) {
-    case LBM_FIELD_IS_BOUNDARY: ((uint8_t*)out)[x] = bnd[x]; break;
-    case LBM_FIELD_NEAR_WALL: ((uint8_t*)out)[x] = (!solid && cls < 10) ? 1 : 0; break;
-    case LBM_FIELD_PARTIAL: ((uint8_t*)out)[x] = (m & M_PARTIAL) ? 1 : 0; break;
-    case LBM_FIELD_SOLID: ((uint8_t*)out)[x] = solid ? 1 : 0; break;
+    case LBM_FIELD_IS_BOUNDARY: *out8 = bnd[x]; break;
+    case LBM_FIELD_NEAR_WALL: *out8 = (!solid && cls < 10) ? 1 : 0; break;
+    case LBM_FIELD_PARTIAL: *out8 = (m & M_PARTIAL) ? 1 : 0; break;
+    case LBM_FIELD_SOLID: *out8 = solid ? 1 : 0; break;
     case LBM_FIELD_WALL_WEIGHT: {
       float w = 0.0f;
       if (!solid) {
         const float arg = -(float)(cls <= 11 ? cls : 100) / 2.5f;
         w = (float)exp((double)arg);
       }
-      ((float*)out)[x] = w;
+      *outf = w;
       break;
     }
   }
@@ -723,12 +789,10 @@ static int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st) {
   switch (which) {
     case LBM_FIELD_PHI: a.kind = OBS_PHI; break;
     case LBM_FIELD_PHI_RAW:
-      HIPCHK(hipMemcpyAsync(dst, a.phi_cur, N * s->esz, hipMemcpyDeviceToDevice, st));
-      return LBM_OK;
+      return to_public(s, a.phi_cur, dst, s->esz, 1, st);
     case LBM_FIELD_RHO: a.kind = OBS_RHO; break;
     case LBM_FIELD_U:
-      HIPCHK(hipMemcpyAsync(dst, s->u, (size_t)s->dim * N * s->esz, hipMemcpyDeviceToDevice, st));
-      return LBM_OK;
+      return to_public(s, s->u, dst, s->esz, s->dim, st);
     case LBM_FIELD_MU:
       a.kind = OBS_MU;
       a.init_lap = s->k == 0 ? 1 : 0;
@@ -745,11 +809,10 @@ static int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st) {
       a.local = s->prev_local ? 1 : 0;
       break;
     case LBM_FIELD_WALL_NORMAL:
-      HIPCHK(hipMemcpyAsync(dst, s->normal, (size_t)s->dim * N * 4, hipMemcpyDeviceToDevice, st));
-      return LBM_OK;
+      return to_public(s, s->normal, dst, 4, s->dim, st);
     case LBM_FIELD_IS_BOUNDARY: case LBM_FIELD_NEAR_WALL: case LBM_FIELD_WALL_WEIGHT:
     case LBM_FIELD_PARTIAL: case LBM_FIELD_SOLID:
-      k_geom_obs<BS><<<s->nblocks, BS, 0, st>>>(s->N, s->meta, s->bnd, which, dst);
+      k_geom_obs<BS><<<s->nblocks, BS, 0, st>>>(s->g, s->dim, s->meta, s->bnd, which, dst);
       HIPCHK(hipGetLastError());
       return LBM_OK;
     default:
@@ -773,11 +836,20 @@ int lbm_get_field(lbm_solver* s, int which, void* dst_dev, void* stream) {
 template <typename T, int DIM>
 static int gradlap_impl(lbm_solver* s, const void* phi, void* grad, void* lap, hipStream_t st) {
   const Phys<T> ph = make_phys<T, DIM>(s);
-  k_gradlap<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(s->g, ph, (const T*)phi,
-                                                   s->geom ? s->meta : nullptr, s->wmask,
-                                                   s->normal, (T*)grad, (T*)lap);
-  HIPCHK(hipGetLastError());
-  return LBM_OK;
+  // phi arrives in the public order: reorder into a temporary first
+  void* tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, (size_t)s->N * sizeof(T)));
+  int rc = to_internal(s, phi, tmp, sizeof(T), 1, st);
+  if (!rc) {
+    k_gradlap<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(s->g, ph, (const T*)tmp,
+                                                     s->geom ? s->meta : nullptr, s->wmask,
+                                                     s->normal, (T*)grad, (T*)lap);
+    if (hipGetLastError() != hipSuccess) rc = fail(LBM_EHIP, "k_gradlap launch failed");
+  }
+  const hipError_t e = hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  if (!rc && e != hipSuccess) rc = fail(LBM_EHIP, "stream synchronize failed");
+  return rc;
 }
 
 int lbm_grad_lap(lbm_solver* s, const void* phi_dev, void* grad_dev, void* lap_dev,
@@ -981,8 +1053,10 @@ static int gen_case(lbm_batch* b, lbm_solver* s, const lbm_case_spec& sp, hipStr
       s->g, c[0], c[1], c[2], sp.radius, s->p.xi, (float)sp.u_impact[0],
       (float)sp.u_impact[1], (float)sp.u_impact[2], phi, u);
   HIPCHK(hipGetLastError());
-  int rc = lbm_set_geometry(s, solid, kind == SUB_FLAT ? (const void*)frac32 : (const void*)frac64,
-                            kind == SUB_FLAT ? LBM_FRAC_F32 : LBM_FRAC_F64, st);
+  // the generated substrate is in the internal order already
+  HIPCHK(hipMemcpyAsync(s->solid, solid, N, hipMemcpyDeviceToDevice, st));
+  int rc = set_geometry_internal(s, kind == SUB_FLAT ? (const void*)frac32 : (const void*)frac64,
+                                 kind == SUB_FLAT ? LBM_FRAC_F32 : LBM_FRAC_F64, st);
   if (rc) return rc;
   if (s->esz == 8) {
     // fp64 runs take fp32-quantised inputs promoted to fp64, as the

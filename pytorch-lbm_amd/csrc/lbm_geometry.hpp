@@ -60,7 +60,7 @@ __global__ __launch_bounds__(BS) void k_normals_frac(Grid g, const FT* __restric
   FT gr[DIM];
   gr[0] = npgrad<FT>(frac, x, c.x, g.nx, g.sx);
   gr[1] = npgrad<FT>(frac, x, c.y, g.ny, g.sy);
-  if constexpr (DIM == 3) gr[2] = npgrad<FT>(frac, x, c.z, g.nz, 1);
+  if constexpr (DIM == 3) gr[2] = npgrad<FT>(frac, x, c.z, g.nz, g.sz);
   // Python's sum() starts from integer 0: ((0 + g0^2) + g1^2) + g2^2
   FT s = gr[0] * gr[0];
 #pragma unroll
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(BS) void k_normals_discrete(Grid g,
       const int pz = DIM == 3 ? c.z + E<DIM>(i, 2) : 0;
       if (px < 0 || px >= g.nx || py < 0 || py >= g.ny) continue;
       if (DIM == 3 && (pz < 0 || pz >= g.nz)) continue;
-      const long long y = (long long)px * g.sx + (long long)py * g.sy + pz;
+      const long long y = int_index<DIM>(g, px, py, pz);
       if (solid[y]) {
 #pragma unroll
         for (int d = 0; d < DIM; ++d) v[d] = v[d] - (double)E<DIM>(i, d);
@@ -188,21 +188,23 @@ __global__ __launch_bounds__(BS) void k_meta(Grid g, const uint8_t* __restrict__
 }
 
 // Topmost solid index per column (last axis), -1 if the column has none.
+// Columns are numbered as in the public arrays: x * ny + y (3D), x (2D).
 template <int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_surface(Grid g, const uint8_t* __restrict__ solid,
                                                 int16_t* __restrict__ surf) {
   const long long col = (long long)blockIdx.x * BS + threadIdx.x;
   if constexpr (DIM == 3) {
     if (col >= (long long)g.nx * g.ny) return;
+    const int cx = (int)(col / g.ny), cy = (int)(col % g.ny);
     int h = -1;
     for (int z = g.nz - 1; z >= 0; --z)
-      if (solid[col * g.nz + z]) { h = z; break; }
+      if (solid[int_index<3>(g, cx, cy, z)]) { h = z; break; }
     surf[col] = (int16_t)h;
   } else {
     if (col >= g.nx) return;
     int h = -1;
     for (int y = g.ny - 1; y >= 0; --y)
-      if (solid[col * g.ny + y]) { h = y; break; }
+      if (solid[int_index<2>(g, (int)col, y, 0)]) { h = y; break; }
     surf[col] = (int16_t)h;
   }
 }
@@ -214,7 +216,9 @@ namespace lbm {
 // ---------------------------------------------------------------------------
 // Device-side case generation (geometry/substrate.py:180-267 and
 // lbm/fe_droplet.py:22-112 of the reference), used by the batch engine so a
-// case is built without host round trips.  Same formulas and evaluation order
+// case is built without host round trips.  The substrate is written in the
+// internal node order (it feeds the geometry precompute directly), the droplet
+// in the public order (it goes through init_fields).  Same formulas and evaluation order
 // as the numpy generators: fraction = clip(signed distance + 0.5, 0, 1) in
 // fp64 (fp32 for flat), solid = fraction >= 0.5; droplet phi =
 // clip(0.5 + 0.5 tanh(2 (R - r) / xi), 0, 1) evaluated in fp64 and rounded to
@@ -284,10 +288,12 @@ __global__ __launch_bounds__(BS) void k_gen_droplet(Grid g, double cx0, double c
   double C = 0.5 + 0.5 * tanh(2.0 * (R - r) / xi);
   C = C < 0.0 ? 0.0 : (C > 1.0 ? 1.0 : C);
   const float Cf = (float)C;
-  phi[x] = Cf;
-  u[x] = ux * Cf;
-  u[g.N + x] = uy * Cf;
-  if (DIM == 3) u[2 * g.N + x] = uz * Cf;
+  // public order: these arrays go through init_fields like user input
+  const uint32_t e = ext_index<DIM>(g, c);
+  phi[e] = Cf;
+  u[e] = ux * Cf;
+  u[g.N + e] = uy * Cf;
+  if (DIM == 3) u[2 * g.N + e] = uz * Cf;
 }
 
 }  // namespace lbm

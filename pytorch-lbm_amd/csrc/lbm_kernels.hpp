@@ -423,10 +423,12 @@ __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __rest
   const uint32_t x = blockIdx.x * BS + threadIdx.x;
   double v = 0.0;
   if (x < N) {
-    const T p = clamp01(phi_in[x]);
+    // phi_in, u_in in the public (last axis fastest) order
+    const uint32_t e = ext_index<DIM>(g, decompose<DIM>(g, x));
+    const T p = clamp01(phi_in[e]);
     T uu[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) uu[d] = u_in ? u_in[(long long)d * N + x] : T(0);
+    for (int d = 0; d < DIM; ++d) uu[d] = u_in ? u_in[(long long)d * N + e] : T(0);
     phi[x] = p;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) u[(long long)d * N + x] = uu[d];
@@ -880,6 +882,7 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
   if (x >= N) return;
   const uint8_t m = a.meta[x];
   const Coord<DIM> c = decompose<DIM>(g, x);
+  const uint32_t ex = ext_index<DIM>(g, c);   // outputs in the public order
   switch (a.kind) {
     case OBS_PHI: {
       T p = a.phi_cur[x];
@@ -890,12 +893,12 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
           pc = clamp01(pc + ph.s12_r * (ph.s12_t - pc));
         p = pc;
       }
-      a.out[x] = p;
+      a.out[ex] = p;
       break;
     }
     case OBS_RHO: {
       const T p = a.phi_cur[x];
-      a.out[x] = p * ph.rho_l + (T(1) - p) * ph.rho_g;
+      a.out[ex] = p * ph.rho_l + (T(1) - p) * ph.rho_g;
       break;
     }
     case OBS_MU: {
@@ -908,7 +911,7 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
       if (bnd) wm = a.wmask[x];
       T gr[DIM], lap;
       stencil<T, DIM>(ph, g, a.phi_cur, x, c, p, wm, n32, gr, &lap, a.init_lap != 0);
-      a.out[x] = chem_pot<T>(ph, p, lap, bnd);
+      a.out[ex] = chem_pot<T>(ph, p, lap, bnd);
       break;
     }
     case OBS_P:
@@ -923,21 +926,22 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
       else
         pull<T, S, DIM, false>(bF, (uint32_t)N, x, c, g, m, e, a.local != 0, bR, ph, f);
       if (a.kind == OBS_P) {
-        a.out[x] = qsum<Q>(f);
+        a.out[ex] = qsum<Q>(f);
         break;
       }
       if (a.kind == OBS_F && a.scaled && !a.local)
         scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot], m, (m & M_TOP) ? a.phi_cur[x] : T(0),
                         nullptr);
 #pragma unroll
-      for (int i = 0; i < Q; ++i) a.out[(long long)i * N + x] = f[i];
+      for (int i = 0; i < Q; ++i) a.out[(long long)i * N + ex] = f[i];
       break;
     }
   }
 }
 
 // Gradient and Laplacian of an arbitrary phi (the private operator the
-// reference's MMS script calls).
+// reference's MMS script calls); phi in internal order, outputs in the public
+// order.
 template <typename T, int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_gradlap(Grid g, Phys<T> ph, const T* __restrict__ phi,
                                                 const uint8_t* __restrict__ meta,
@@ -955,9 +959,29 @@ __global__ __launch_bounds__(BS) void k_gradlap(Grid g, Phys<T> ph, const T* __r
   for (int d = 0; d < DIM; ++d) n32[d] = bnd ? normal[(long long)d * N + x] : 0.0f;
   T gr[DIM], l;
   stencil<T, DIM>(ph, g, phi, x, c, phi[x], bnd ? wmask[x] : 0u, n32, gr, &l, false);
+  const uint32_t ex = ext_index<DIM>(g, c);
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) grad[(long long)d * N + x] = gr[d];
-  lap[x] = l;
+  for (int d = 0; d < DIM; ++d) grad[(long long)d * N + ex] = gr[d];
+  lap[ex] = l;
+}
+
+// Reordering between the public layout (last axis fastest) and the internal
+// one, for `comps` stacked arrays of N elements each.
+template <typename E, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_to_internal(Grid g, const E* __restrict__ pub,
+                                                    E* __restrict__ in, int comps) {
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  const uint32_t ex = ext_index<DIM>(g, decompose<DIM>(g, x));
+  for (int k = 0; k < comps; ++k) in[(long long)k * g.N + x] = pub[(long long)k * g.N + ex];
+}
+template <typename E, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_to_public(Grid g, const E* __restrict__ in,
+                                                  E* __restrict__ pub, int comps) {
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  const uint32_t ex = ext_index<DIM>(g, decompose<DIM>(g, x));
+  for (int k = 0; k < comps; ++k) pub[(long long)k * g.N + ex] = in[(long long)k * g.N + x];
 }
 
 }  // namespace lbm

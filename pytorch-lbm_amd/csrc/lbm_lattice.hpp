@@ -83,14 +83,25 @@ enum : uint8_t {
 };
 __host__ __device__ inline int dist_class(uint8_t m) { return m >> 4; }  // 0..11, 15 = far
 
-// Grid index helpers.  Linear index = (ix*ny + iy)*nz + iz (3D) or ix*ny + iy
-// (2D), last axis fastest, as the reference's (Q, nx, ny[, nz]) tensors.
+// Grid indexing.
+//
+// The public arrays (the reference's (Q, nx, ny[, nz]) tensors) run the last
+// axis -- the wall-normal one -- fastest.  Inside the library nodes are
+// ordered with that axis SLOWEST: 3D index = (z * nx + x) * ny + y, 2D index =
+// y * nx + x.  A wave of 64 consecutive nodes then lies in one layer parallel
+// to the substrate, so walls, solid and bulk fluid split the waves by layer
+// instead of every wave holding a piece of a column from the floor up, and the
+// wall-only work (bounce-back, wetting ghosts, gradient correction) runs in
+// the few waves that hold wall nodes.  ext_index() maps a node to its position
+// in the public arrays; every transfer across the API boundary goes through it.
 struct Grid {
   int nx, ny, nz;        // nz == 1 for 2D
-  int sx, sy;            // strides of x and y
+  int sx, sy, sz;        // internal strides of the x, y, z axes
+  int wx, wy, wz;        // n_d * s_d: the jump of a periodic wrap along each axis
   long long N;
-  // magic-number division by nz and ny (unsigned, exact for 0 <= v < 2^31)
-  uint32_t mz, sz_shift, my, sy_shift;
+  // magic-number division by the fastest axis length (n1) and by the middle
+  // one (n2; 3D only); exact for 0 <= v < 2^31
+  uint32_t n1, m1, sh1, n2, m2, sh2;
 };
 
 __device__ __forceinline__ uint32_t fdiv(uint32_t v, uint32_t m, uint32_t sh, uint32_t d) {
@@ -106,44 +117,82 @@ template <int DIM>
 __device__ __forceinline__ Coord<DIM> decompose(const Grid& g, uint32_t idx) {
   Coord<DIM> c;
   if constexpr (DIM == 3) {
-    uint32_t q = fdiv(idx, g.mz, g.sz_shift, (uint32_t)g.nz);
-    c.z = (int)(idx - q * (uint32_t)g.nz);
-    uint32_t q2 = fdiv(q, g.my, g.sy_shift, (uint32_t)g.ny);
-    c.y = (int)(q - q2 * (uint32_t)g.ny);
-    c.x = (int)q2;
+    const uint32_t q = fdiv(idx, g.m1, g.sh1, g.n1);     // idx / ny
+    c.y = (int)(idx - q * g.n1);
+    const uint32_t q2 = fdiv(q, g.m2, g.sh2, g.n2);      // / nx
+    c.x = (int)(q - q2 * g.n2);
+    c.z = (int)q2;
   } else {
-    uint32_t q = fdiv(idx, g.my, g.sy_shift, (uint32_t)g.ny);
-    c.y = (int)(idx - q * (uint32_t)g.ny);
-    c.x = (int)q;
+    const uint32_t q = fdiv(idx, g.m1, g.sh1, g.n1);     // idx / nx
+    c.x = (int)(idx - q * g.n1);
+    c.y = (int)q;
     c.z = 0;
   }
   return c;
 }
 
-// Linear offset of the periodic neighbour at c + (dx, dy, dz).
+template <int DIM>
+__device__ __forceinline__ uint32_t int_index(const Grid& g, int x, int y, int z) {
+  return (uint32_t)(x * g.sx + y * g.sy + (DIM == 3 ? z * g.sz : 0));
+}
+
+// Position of a node in the public (last axis fastest) arrays.
+template <int DIM>
+__device__ __forceinline__ uint32_t ext_index(const Grid& g, const Coord<DIM>& c) {
+  if constexpr (DIM == 3) return (uint32_t)((c.x * g.ny + c.y) * g.nz + c.z);
+  else return (uint32_t)(c.x * g.ny + c.y);
+}
+
+// Periodic neighbour offsets of one node along each axis: p[d] to the +1
+// neighbour, m[d] to the -1 neighbour (a wrap at the faces).  Computed once per
+// node; the offset of any lattice direction is then a sum of at most DIM of
+// them.
+template <int DIM> struct AxisOff {
+  int p[DIM], m[DIM];
+};
+
+template <int DIM>
+__device__ __forceinline__ AxisOff<DIM> axis_offsets(const Grid& g, const Coord<DIM>& c) {
+  AxisOff<DIM> o;
+  const int cc[3] = {c.x, c.y, c.z};
+  const int n[3] = {g.nx, g.ny, g.nz};
+  const int st[3] = {g.sx, g.sy, g.sz};
+  const int w[3] = {g.wx, g.wy, g.wz};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    o.p[d] = cc[d] == n[d] - 1 ? st[d] - w[d] : st[d];
+    o.m[d] = cc[d] == 0 ? w[d] - st[d] : -st[d];
+  }
+  return o;
+}
+
+// Offset of x + s * e_i (s = +1 or -1) from precomputed axis offsets.
+template <int DIM>
+__device__ __forceinline__ int dir_offset(const AxisOff<DIM>& o, int i, int s) {
+  int off = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int e = E<DIM>(i, d) * s;
+    if (e > 0) off += o.p[d];
+    else if (e < 0) off += o.m[d];
+  }
+  return off;
+}
+
+// Linear offset of the periodic neighbour at c + (dx, dy, dz), |d*| <= 1.
 template <int DIM>
 __device__ __forceinline__ int nb_offset(const Grid& g, const Coord<DIM>& c, int dx, int dy,
                                          int dz) {
+  const int cc[3] = {c.x, c.y, c.z};
+  const int dd[3] = {dx, dy, dz};
+  const int n[3] = {g.nx, g.ny, g.nz};
+  const int st[3] = {g.sx, g.sy, g.sz};
+  const int w[3] = {g.wx, g.wy, g.wz};
   int off = 0;
-  if (dx) {
-    int xx = c.x + dx;
-    off += dx * g.sx;
-    if (xx < 0) off += g.nx * g.sx;
-    else if (xx >= g.nx) off -= g.nx * g.sx;
-  }
-  if (dy) {
-    int yy = c.y + dy;
-    off += dy * g.sy;
-    if (yy < 0) off += g.ny * g.sy;
-    else if (yy >= g.ny) off -= g.ny * g.sy;
-  }
-  if constexpr (DIM == 3) {
-    if (dz) {
-      int zz = c.z + dz;
-      off += dz;
-      if (zz < 0) off += g.nz;
-      else if (zz >= g.nz) off -= g.nz;
-    }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    if (dd[d] > 0) off += cc[d] == n[d] - 1 ? st[d] - w[d] : st[d];
+    else if (dd[d] < 0) off += cc[d] == 0 ? w[d] - st[d] : -st[d];
   }
   return off;
 }

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(BS) void k_contact(Grid g, Phys<T> ph, const T* __r
     const int h = surf[col];
     const int z = h + 1;
     if (z < g.nz) {
-      const long long x = col * g.nz + z;
+      const long long x = int_index<DIM>(g, (int)(col / g.ny), (int)(col % g.ny), z);
       const uint8_t m = meta[x];
       const T cf = scaled ? (T)ctrl->c[slot] : T(1);
       const T p = phi_observed<T, DIM>(ph, phi[x], m, cf, scaled);
