@@ -146,7 +146,15 @@ __device__ __forceinline__ void bput(const Buf& b, uint32_t N, int i, uint32_t i
 __host__ __device__ constexpr long long padded_nodes(long long N) { return (N + 63) & ~63LL; }
 
 template <int Q, typename S> __device__ __forceinline__ uint32_t dvoff(uint32_t idx) {
-  return ((idx >> 6) * (uint32_t)(Q * 64) + (idx & 63u)) * (uint32_t)sizeof(S);
+  // = ((idx / 64) * Q * 64 + idx % 64) * sizeof(S), with a full-rate 24-bit
+  // multiply (idx / 64 < 2^24; the product stays below 2^32 because a
+  // distribution set is smaller than 4 GiB)
+  return __umul24(idx >> 6, (uint32_t)((Q - 1) * 64 * sizeof(S))) + idx * (uint32_t)sizeof(S);
+}
+// load of direction i at the node whose dvoff is vo
+template <int Q, typename T, typename S>
+__device__ __forceinline__ T dload(const Buf& b, int i, uint32_t vo) {
+  return Stor<S>::template to<T>(BufIO<S>::ld(b, vo, (uint32_t)i * 64u * (uint32_t)sizeof(S)));
 }
 template <int Q, typename T, typename S>
 __device__ __forceinline__ T dget(const Buf& b, int i, uint32_t idx) {
@@ -236,24 +244,50 @@ __device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int n
 }
 
 // ---------------------------------------------------------------------------
-// Pull of one distribution set at node x.
+// Neighbour indices of one node: i[k] = index of x + e_k (periodic), i[0] = x.
+// The pull source of direction k is x - e_k = i[OPP(k)]; the phi stencil reads
+// i[k].  Built once per node from the six wrapped axis offsets.
+// ---------------------------------------------------------------------------
+template <int DIM> struct Nbrs {
+  uint32_t i[Lat<DIM>::Q];
+};
+
+template <int DIM>
+__device__ __forceinline__ Nbrs<DIM> neighbours(const Grid& g, uint32_t x, const Coord<DIM>& c) {
+  constexpr int Q = Lat<DIM>::Q;
+  const AxisOff<DIM> o = axis_offsets<DIM>(g, c);
+  Nbrs<DIM> n;
+  n.i[0] = x;
+#pragma unroll
+  for (int k = 1; k < Q; ++k) n.i[k] = (uint32_t)((int)x + dir_offset<DIM>(o, k, 1));
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// Pull of the distribution sets at node x: f from F and, with WITH_G, g from G
+// (same layout, so both share the per-direction source offsets).
 //
 // DEV (16-bit f storage): slot (i, y) holds f*_i(y) - w_i phi_ref(y + e_i),
 // phi_ref being the start-of-step phi of the step that wrote it, so a fluid
 // pull adds back w_i phi_ref(x) of the pulling node itself; bounce-back and
 // penalization slots (i, x) read at solid/partial x add w_i phi_ref(x - e_i).
+// g is stored as is.
 // ---------------------------------------------------------------------------
-template <typename T, typename S, int DIM, bool DEV>
-__device__ __forceinline__ void pull(const Buf& F, uint32_t N, uint32_t x, const Coord<DIM>& c,
-                                     const Grid& g, uint8_t m, T eps, bool local,
-                                     const Buf& pref, const Phys<T>& ph,
-                                     T (&f)[Lat<DIM>::Q]) {
+template <typename T, typename S, int DIM, bool DEV, bool WITH_G>
+__device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, const Nbrs<DIM>& nb,
+                                     uint8_t m, T eps, bool local, const Buf& pref,
+                                     const Phys<T>& ph, T (&f)[Lat<DIM>::Q],
+                                     T (&gg)[Lat<DIM>::Q]) {
   constexpr int Q = Lat<DIM>::Q;
+  const uint32_t vx = dvoff<Q, S>(x);
   if (local) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = dget<Q, T, S>(F, i, x);
+    for (int i = 0; i < Q; ++i) {
+      f[i] = dload<Q, T, S>(F, i, vx);
+      if constexpr (WITH_G) gg[i] = dload<Q, T, S>(G, i, vx);
+    }
     if constexpr (DEV) {
-      const T r = bget<T, T>(pref, N, 0, x);
+      const T r = bget<T, T>(pref, 0, 0, x);
 #pragma unroll
       for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.w[i] * r;
     }
@@ -262,23 +296,20 @@ __device__ __forceinline__ void pull(const Buf& F, uint32_t N, uint32_t x, const
   if (m & M_SOLID) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      f[i] = dget<Q, T, S>(F, OPP<DIM>(i), x);
-      if constexpr (DEV) {
-        const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
-                                       DIM == 3 ? -E<DIM>(i, 2) : 0);
-        f[i] = f[i] + ph.w[i] * bget<T, T>(pref, N, 0, (uint32_t)((int)x + off));
-      }
+      f[i] = dload<Q, T, S>(F, OPP<DIM>(i), vx);
+      if constexpr (WITH_G) gg[i] = dload<Q, T, S>(G, OPP<DIM>(i), vx);
+      if constexpr (DEV) f[i] = f[i] + ph.w[i] * bget<T, T>(pref, 0, 0, nb.i[OPP<DIM>(i)]);
     }
     return;
   }
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
-                                   DIM == 3 ? -E<DIM>(i, 2) : 0);
-    f[i] = dget<Q, T, S>(F, i, (uint32_t)((int)x + off));
+    const uint32_t vs = dvoff<Q, S>(nb.i[OPP<DIM>(i)]);
+    f[i] = dload<Q, T, S>(F, i, vs);
+    if constexpr (WITH_G) gg[i] = dload<Q, T, S>(G, i, vs);
   }
   if constexpr (DEV) {
-    const T r = bget<T, T>(pref, N, 0, x);
+    const T r = bget<T, T>(pref, 0, 0, x);
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.w[i] * r;
   }
@@ -286,13 +317,13 @@ __device__ __forceinline__ void pull(const Buf& F, uint32_t N, uint32_t x, const
     const T a = T(1) - eps;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      T o = dget<Q, T, S>(F, OPP<DIM>(i), x);
-      if constexpr (DEV) {
-        const int off = nb_offset<DIM>(g, c, -E<DIM>(i, 0), -E<DIM>(i, 1),
-                                       DIM == 3 ? -E<DIM>(i, 2) : 0);
-        o = o + ph.w[i] * bget<T, T>(pref, N, 0, (uint32_t)((int)x + off));
-      }
+      T o = dload<Q, T, S>(F, OPP<DIM>(i), vx);
+      if constexpr (DEV) o = o + ph.w[i] * bget<T, T>(pref, 0, 0, nb.i[OPP<DIM>(i)]);
       f[i] = a * f[i] + eps * o;
+      if constexpr (WITH_G) {
+        const T og = dload<Q, T, S>(G, OPP<DIM>(i), vx);
+        gg[i] = a * gg[i] + eps * og;
+      }
     }
   }
 }
@@ -472,10 +503,12 @@ __device__ __forceinline__ T phi_node(const Grid& g, const Phys<T>& ph, const S*
   const long long N = g.N;
   const uint8_t m = meta[x];
   const Coord<DIM> c = decompose<DIM>(g, x);
+  const Nbrs<DIM> nb = neighbours<DIM>(g, x, c);
   const T e = (m & M_PARTIAL) ? eps[x] : T(0);
-  T f[Q];
+  T f[Q], unused[Q];
   const Buf bF = dist_buf<DIM>(Fin, N), bP = field_buf(phi_prev, N);
-  pull<T, S, DIM, !std::is_same<T, S>::value>(bF, (uint32_t)N, x, c, g, m, e, local, bP, ph, f);
+  pull<T, S, DIM, !std::is_same<T, S>::value, false>(bF, bF, x, nb, m, e, local, bP, ph, f,
+                                                     unused);
   if (!local) scale_f<T, DIM>(ph, f, cprev, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr);
   T p = clamp01(qsum<Q>(f));
   if (m & M_SOLID) p = T(0);
@@ -526,9 +559,9 @@ __device__ __forceinline__ void finalize_mass(const double* __restrict__ partial
 // ---------------------------------------------------------------------------
 template <typename T, int DIM>
 __device__ __forceinline__ void stencil(const Phys<T>& ph, const Grid& g,
-                                        const T* __restrict__ phi, uint32_t x,
-                                        const Coord<DIM>& c, T p, uint32_t wm, const float* n32,
-                                        T* grad, T* lap, bool init_lap) {
+                                        const T* __restrict__ phi, const Nbrs<DIM>& nb, T p,
+                                        uint32_t wm, const float* n32, T* grad, T* lap,
+                                        bool init_lap) {
   constexpr int Q = Lat<DIM>::Q;
   const Buf bphi = field_buf(phi, g.N);
 #pragma unroll
@@ -536,9 +569,7 @@ __device__ __forceinline__ void stencil(const Phys<T>& ph, const Grid& g,
   T l = T(0);
 #pragma unroll
   for (int i = 1; i < Q; ++i) {
-    const int off = nb_offset<DIM>(g, c, E<DIM>(i, 0), E<DIM>(i, 1),
-                                   DIM == 3 ? E<DIM>(i, 2) : 0);
-    T psi = bget<T, T>(bphi, (uint32_t)g.N, 0, (uint32_t)((int)x + off));
+    T psi = bget<T, T>(bphi, 0, 0, nb.i[i]);
     if (wm & (1u << i)) {
       // n . e_i accumulated in fp32 from zero (lbm/fe_wetting.py:237-239)
       float nde = 0.0f;
@@ -647,22 +678,20 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   const bool solid = m & M_SOLID;
   const bool bnd = m & M_BOUNDARY;
   const Coord<DIM> c = decompose<DIM>(g, x);
+  const Nbrs<DIM> nb = neighbours<DIM>(g, x, c);
   const T eps = (m & M_PARTIAL) ? a.eps[x] : T(0);
 
-  T f[Q], gg[Q];
+  // g is pulled first and f only for the final f collision, after g* and u
+  // are stored: f's 19 values are then never live together with g's, which
+  // keeps the register footprint (and so the occupancy) of this pass down.
+  T gg[Q], unused[Q];
   const uint32_t N32 = (uint32_t)N;
   const Buf bFi = dist_buf<DIM>(a.Fin, N), bGi = dist_buf<DIM>(a.Gin, N);
   const Buf bPp = field_buf(a.phi_prev, N), bPc = field_buf(a.phi_cur, N);
   const Buf bU = field_buf(a.u, N, DIM);
-  pull<T, S, DIM, DEV>(bFi, N32, x, c, g, m, eps, a.local != 0, bPp, ph, f);
-  pull<T, S, DIM, false>(bGi, N32, x, c, g, m, eps, a.local != 0, bPp, ph, gg);
+  pull<T, S, DIM, false, false>(bGi, bGi, x, nb, m, eps, a.local != 0, bPp, ph, gg, unused);
   T pprev = T(0);
-  if (!a.local) {
-    if ((m & M_TOP) || a.phiu_valid) pprev = a.phi_prev[x];
-    scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot_prev], m, pprev, nullptr);
-  } else if (a.phiu_valid) {
-    pprev = a.phi_prev[x];
-  }
+  if (((m & M_TOP) && !a.local) || a.phiu_valid) pprev = a.phi_prev[x];
 
   // S1/S2: phi, rho, stencil, mu, correction
   const T p = at(a.phi_cur, x);
@@ -679,7 +708,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
     for (int d = 0; d < DIM; ++d) n32[d] = 0.0f;
   }
   T gr[DIM], lap;
-  stencil<T, DIM>(ph, g, a.phi_cur, x, c, p, wm, n32, gr, &lap, false);
+  stencil<T, DIM>(ph, g, a.phi_cur, nb, p, wm, n32, gr, &lap, false);
   const T mu = chem_pot<T>(ph, p, lap, bnd && ph.wet != WET_NONE);
   if (ph.corr_on && bnd) {
     const T an = a.ampf ? a.ampf[x] : T(0);
@@ -798,49 +827,72 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   }
   const Buf bFo = dist_buf<DIM>(a.Fout, N), bGo = dist_buf<DIM>(a.Gout, N);
   const T usq_n = divk(vsq<DIM>(un), ph.two_cs2, ph.rcp_two_cs2);
-  const T rc = rho_eff * ph.cs2;
-  const T rrc = T(1) / rc;
   const uint32_t vx = dvoff<Q, S>(x);
-  T ec[Q], fsrc[Q], G[Q];
+  T ec[Q];
   ediv<T, DIM>(ph, un, ph.cs2, ph.rcp_cs2, ec);
-  // source and Guo terms: odd in e, so opposite directions are exact negatives
-  fsrc[0] = T(0);
-  G[0] = T(0);
+  // g collision with the Guo term (odd in e: opposite directions are exact
+  // negatives), then g* and u are stored
+  {
+    const T rc = rho_eff * ph.cs2;
+    const T rrc = T(1) / rc;
+    T G[Q];
+    G[0] = T(0);
 #pragma unroll
-  for (int i = 1; i < Q; ++i) {
-    if (OPP<DIM>(i) > i) {
-      fsrc[i] = divk((ph.acoef * ph.w[i]) * edot<DIM>(i, src), ph.cs2, ph.rcp_cs2);
-      G[i] = divk((ns * ph.w[i]) * edot<DIM>(i, F), rc, rrc);
-    }
-  }
+    for (int i = 1; i < Q; ++i)
+      if (OPP<DIM>(i) > i) G[i] = divk((ns * ph.w[i]) * edot<DIM>(i, F), rc, rrc);
 #pragma unroll
-  for (int i = 1; i < Q; ++i) {
-    if (OPP<DIM>(i) < i) {
-      fsrc[i] = -fsrc[OPP<DIM>(i)];
-      G[i] = -G[OPP<DIM>(i)];
-    }
-  }
+    for (int i = 1; i < Q; ++i)
+      if (OPP<DIM>(i) < i) G[i] = -G[OPP<DIM>(i)];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const T fe = feq_of<T>(ph, i, p, ec[i], usq_n);
-    const T ge = geq_of<T>(ph, i, P, ec[i], usq_n);
-    const T fs = (f[i] - ph.om_f * (f[i] - fe)) + fsrc[i];
-    const T gs = (gg[i] - omg * (gg[i] - ge)) + G[i];
-    T fst = fs;
-    if constexpr (DEV) {
-      const int off = nb_offset<DIM>(g, c, E<DIM>(i, 0), E<DIM>(i, 1),
-                                     DIM == 3 ? E<DIM>(i, 2) : 0);
-      fst = fs - ph.w[i] * bget<T, T>(bPc, N32, 0, (uint32_t)((int)x + off));
+    for (int i = 0; i < Q; ++i) {
+      const T ge = geq_of<T>(ph, i, P, ec[i], usq_n);
+      const T gs = (gg[i] - omg * (gg[i] - ge)) + G[i];
+      dput<Q, T, S>(bGo, i, vx, gs);
     }
-    dput<Q, T, S>(bFo, i, vx, fst);
-    dput<Q, T, S>(bGo, i, vx, gs);
   }
 #pragma unroll
   for (int d = 0; d < DIM; ++d) bput<T, T>(bU, N32, d, x, un[d]);
+
+  // f: pull, mass factor and boundary relax of the previous step, collision
+  // with the interface source (odd in e as well)
+  // the neighbour indices are rebuilt here rather than kept live from the
+  // top (18 registers across the whole node); the empty asm hides x's
+  // identity so the compiler cannot merge the two computations
+  uint32_t x2 = x;
+  asm volatile("" : "+v"(x2));
+  const Nbrs<DIM> nb2 = neighbours<DIM>(g, x2, decompose<DIM>(g, x2));
+  T f[Q];
+  pull<T, S, DIM, DEV, false>(bFi, bFi, x, nb2, m, eps, a.local != 0, bPp, ph, f, unused);
+  if (!a.local) scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot_prev], m, pprev, nullptr);
+  T fsrc[Q];
+  fsrc[0] = T(0);
+#pragma unroll
+  for (int i = 1; i < Q; ++i)
+    if (OPP<DIM>(i) > i)
+      fsrc[i] = divk((ph.acoef * ph.w[i]) * edot<DIM>(i, src), ph.cs2, ph.rcp_cs2);
+#pragma unroll
+  for (int i = 1; i < Q; ++i)
+    if (OPP<DIM>(i) < i) fsrc[i] = -fsrc[OPP<DIM>(i)];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const T fe = feq_of<T>(ph, i, p, ec[i], usq_n);
+    const T fs = (f[i] - ph.om_f * (f[i] - fe)) + fsrc[i];
+    T fst = fs;
+    if constexpr (DEV) fst = fs - ph.w[i] * bget<T, T>(bPc, 0, 0, nb2.i[i]);
+    dput<Q, T, S>(bFo, i, vx, fst);
+  }
 }
 
+// Occupancy target of pass B (waves per SIMD); a build-time tuning knob.
+#ifdef LBM_COLLIDE_WPE
+#define LBM_COLLIDE_ATTR __attribute__((amdgpu_waves_per_eu(LBM_COLLIDE_WPE)))
+#else
+#define LBM_COLLIDE_ATTR
+#endif
+
 template <typename T, typename S, int DIM, int BS>
-__global__ __launch_bounds__(BS) void k_collide(CollideArgs<T, S, DIM> a, Phys<T> ph) {
+__global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_collide(CollideArgs<T, S, DIM> a,
+                                                                 Phys<T> ph) {
   if (blockIdx.x == 0) finalize_mass<BS>(a.partials, a.nparts, a.ctrl_w, a.slot_new);
   const uint32_t x = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (x >= a.g.N) return;
@@ -910,7 +962,8 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
       for (int d = 0; d < DIM; ++d) n32[d] = bnd ? a.normal[(long long)d * N + x] : 0.0f;
       if (bnd) wm = a.wmask[x];
       T gr[DIM], lap;
-      stencil<T, DIM>(ph, g, a.phi_cur, x, c, p, wm, n32, gr, &lap, a.init_lap != 0);
+      stencil<T, DIM>(ph, g, a.phi_cur, neighbours<DIM>(g, x, c), p, wm, n32, gr, &lap,
+                      a.init_lap != 0);
       a.out[ex] = chem_pot<T>(ph, p, lap, bnd);
       break;
     }
@@ -918,13 +971,14 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
     case OBS_F:
     case OBS_G: {
       const T e = (m & M_PARTIAL) ? a.eps[x] : T(0);
-      T f[Q];
+      T f[Q], unused[Q];
       const Buf bF = dist_buf<DIM>(a.Fsrc, N), bR = field_buf(a.pref, N);
+      const Nbrs<DIM> nb = neighbours<DIM>(g, x, c);
       if (a.kind == OBS_F)
-        pull<T, S, DIM, !std::is_same<T, S>::value>(bF, (uint32_t)N, x, c, g, m, e,
-                                                    a.local != 0, bR, ph, f);
+        pull<T, S, DIM, !std::is_same<T, S>::value, false>(bF, bF, x, nb, m, e, a.local != 0,
+                                                           bR, ph, f, unused);
       else
-        pull<T, S, DIM, false>(bF, (uint32_t)N, x, c, g, m, e, a.local != 0, bR, ph, f);
+        pull<T, S, DIM, false, false>(bF, bF, x, nb, m, e, a.local != 0, bR, ph, f, unused);
       if (a.kind == OBS_P) {
         a.out[ex] = qsum<Q>(f);
         break;
@@ -958,7 +1012,8 @@ __global__ __launch_bounds__(BS) void k_gradlap(Grid g, Phys<T> ph, const T* __r
 #pragma unroll
   for (int d = 0; d < DIM; ++d) n32[d] = bnd ? normal[(long long)d * N + x] : 0.0f;
   T gr[DIM], l;
-  stencil<T, DIM>(ph, g, phi, x, c, phi[x], bnd ? wmask[x] : 0u, n32, gr, &l, false);
+  stencil<T, DIM>(ph, g, phi, neighbours<DIM>(g, x, c), phi[x], bnd ? wmask[x] : 0u, n32, gr,
+                  &l, false);
   const uint32_t ex = ext_index<DIM>(g, c);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) grad[(long long)d * N + ex] = gr[d];
