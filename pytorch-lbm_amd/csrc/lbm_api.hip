@@ -505,11 +505,15 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 0], st));
     const int src = s->cur, dst = s->cur ^ 1;
     const int pslot = (int)(s->k & 1), nslot = (int)((s->k + 1) & 1);
-#define LAUNCH_A(B)                                                                       \
-  k_phi<T, S, DIM, B><<<s->nbA, B, 0, st>>>(s->g, ph, (const S*)s->Fb[src],                \
-                                            (const T*)s->phi[pslot], (T*)s->phi[nslot],     \
-                                            s->meta, (const T*)s->eps, s->partials, s->ctrl, \
-                                            pslot, s->local ? 1 : 0)
+#define LAUNCH_A(B)                                                                        \
+  if (s->local)                                                                             \
+    k_phi<T, S, DIM, B, true><<<s->nbA, B, 0, st>>>(                                        \
+        s->g, ph, (const S*)s->Fb[src], (const T*)s->phi[pslot], (T*)s->phi[nslot], s->meta, \
+        (const T*)s->eps, s->partials, s->ctrl, pslot);                                      \
+  else                                                                                      \
+    k_phi<T, S, DIM, B, false><<<s->nbA, B, 0, st>>>(                                       \
+        s->g, ph, (const S*)s->Fb[src], (const T*)s->phi[pslot], (T*)s->phi[nslot], s->meta, \
+        (const T*)s->eps, s->partials, s->ctrl, pslot)
     LBM_BS_SWITCH(s->bsA, LAUNCH_A)
 #undef LAUNCH_A
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 1], st));
@@ -535,7 +539,9 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     a.slot_prev = pslot;
     a.local = s->local ? 1 : 0;
     a.phiu_valid = s->phiu_valid ? 1 : 0;
-#define LAUNCH_B(B) k_collide<T, S, DIM, B><<<s->nbB, B, 0, st>>>(a, ph)
+#define LAUNCH_B(B)                                               \
+  if (s->local) k_collide<T, S, DIM, B, true><<<s->nbB, B, 0, st>>>(a, ph); \
+  else k_collide<T, S, DIM, B, false><<<s->nbB, B, 0, st>>>(a, ph)
     LBM_BS_SWITCH(s->bsB, LAUNCH_B)
 #undef LAUNCH_B
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 2], st));

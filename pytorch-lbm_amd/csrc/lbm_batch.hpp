@@ -57,9 +57,11 @@ __global__ __launch_bounds__(BS) void k_phi_batch(const BatchCase<T, S>* __restr
   const int src = pslot;   // buffers alternate with the step count from init
   double v = 0.0;
   if (x < N) {
-    const T cf = cs.local ? T(1) : (T)cs.ctrl->c[pslot];
-    const T p = phi_node<T, S, DIM>(cs.g, cs.ph, cs.F[src], cs.phi[pslot], cs.meta, cs.eps, cf,
-                                    cs.local != 0, x);
+    const T p = cs.local
+                    ? phi_node<T, S, DIM, true>(cs.g, cs.ph, cs.F[src], cs.phi[pslot], cs.meta,
+                                                cs.eps, T(1), x)
+                    : phi_node<T, S, DIM, false>(cs.g, cs.ph, cs.F[src], cs.phi[pslot], cs.meta,
+                                                 cs.eps, (T)cs.ctrl->c[pslot], x);
     at(cs.phi[nslot], x) = p;
     v = (double)p;
   }
@@ -103,7 +105,8 @@ __global__ __launch_bounds__(BS) void k_collide_batch(const BatchCase<T, S>* __r
   a.slot_new = nslot;
   a.local = cs.local;
   a.phiu_valid = cs.phiu_valid;
-  collide_node<T, S, DIM>(a, cs.ph, x);
+  if (cs.local) collide_node<T, S, DIM, true>(a, cs.ph, x);
+  else collide_node<T, S, DIM, false>(a, cs.ph, x);
 }
 
 // After pass B: every case that stepped advances its counter.

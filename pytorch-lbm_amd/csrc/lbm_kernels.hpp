@@ -273,14 +273,17 @@ __device__ __forceinline__ Nbrs<DIM> neighbours(const Grid& g, uint32_t x, const
 // penalization slots (i, x) read at solid/partial x add w_i phi_ref(x - e_i).
 // g is stored as is.
 // ---------------------------------------------------------------------------
-template <typename T, typename S, int DIM, bool DEV, bool WITH_G>
+// LOCAL: the buffers hold post-stream values (first step after init_fields)
+// and are read in place.  It is a template parameter so that the common form
+// has no branch between the geometry-byte load and the pulls.
+template <typename T, typename S, int DIM, bool DEV, bool WITH_G, bool LOCAL>
 __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, const Nbrs<DIM>& nb,
-                                     uint8_t m, T eps, bool local, const Buf& pref,
-                                     const Phys<T>& ph, T (&f)[Lat<DIM>::Q],
+                                     uint8_t m, const T* __restrict__ eps_arr,
+                                     const Buf& pref, const Phys<T>& ph, T (&f)[Lat<DIM>::Q],
                                      T (&gg)[Lat<DIM>::Q]) {
   constexpr int Q = Lat<DIM>::Q;
   const uint32_t vx = dvoff<Q, S>(x);
-  if (local) {
+  if constexpr (LOCAL) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       f[i] = dload<Q, T, S>(F, i, vx);
@@ -293,27 +296,41 @@ __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, con
     }
     return;
   }
-  if (m & M_SOLID) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      f[i] = dload<Q, T, S>(F, OPP<DIM>(i), vx);
-      if constexpr (WITH_G) gg[i] = dload<Q, T, S>(G, OPP<DIM>(i), vx);
-      if constexpr (DEV) f[i] = f[i] + ph.w[i] * bget<T, T>(pref, 0, 0, nb.i[OPP<DIM>(i)]);
-    }
-    return;
-  }
+  // Fluid pulls are issued for every lane before the node's geometry byte is
+  // known (one memory latency instead of two in series); solid lanes then
+  // take bounce-back values instead.  The choice is a select after the
+  // branch, so the compiler cannot sink the fluid loads behind the branch.
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const uint32_t vs = dvoff<Q, S>(nb.i[OPP<DIM>(i)]);
     f[i] = dload<Q, T, S>(F, i, vs);
     if constexpr (WITH_G) gg[i] = dload<Q, T, S>(G, i, vs);
   }
+  // nothing that reads m may be scheduled above the loads
+  __builtin_amdgcn_sched_barrier(0);
+  const bool solid = (m & M_SOLID) != 0;
+  if (solid) {
+    T bf[Lat<DIM>::Q], bg[Lat<DIM>::Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      bf[i] = dload<Q, T, S>(F, OPP<DIM>(i), vx);
+      if constexpr (WITH_G) bg[i] = dload<Q, T, S>(G, OPP<DIM>(i), vx);
+      if constexpr (DEV) bf[i] = bf[i] + ph.w[i] * bget<T, T>(pref, 0, 0, nb.i[OPP<DIM>(i)]);
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      f[i] = bf[i];
+      if constexpr (WITH_G) gg[i] = bg[i];
+    }
+  }
+  if (solid) return;
   if constexpr (DEV) {
     const T r = bget<T, T>(pref, 0, 0, x);
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.w[i] * r;
   }
   if (m & M_PARTIAL) {
+    const T eps = eps_arr[x];
     const T a = T(1) - eps;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -493,43 +510,45 @@ __global__ __launch_bounds__(BS) void k_sum_partials(const double* __restrict__ 
 // ---------------------------------------------------------------------------
 // Pass A: start-of-step phi and its fp64 sum -> mass factor of this step.
 // ---------------------------------------------------------------------------
-template <typename T, typename S, int DIM>
+template <typename T, typename S, int DIM, bool LOCAL>
 __device__ __forceinline__ T phi_node(const Grid& g, const Phys<T>& ph, const S* __restrict__ Fin,
                                       const T* __restrict__ phi_prev,
                                       const uint8_t* __restrict__ meta,
-                                      const T* __restrict__ eps, T cprev, bool local,
-                                      uint32_t x) {
+                                      const T* __restrict__ eps, T cprev, uint32_t x) {
   constexpr int Q = Lat<DIM>::Q;
   const long long N = g.N;
   const uint8_t m = meta[x];
   const Coord<DIM> c = decompose<DIM>(g, x);
   const Nbrs<DIM> nb = neighbours<DIM>(g, x, c);
-  const T e = (m & M_PARTIAL) ? eps[x] : T(0);
   T f[Q], unused[Q];
   const Buf bF = dist_buf<DIM>(Fin, N), bP = field_buf(phi_prev, N);
-  pull<T, S, DIM, !std::is_same<T, S>::value, false>(bF, bF, x, nb, m, e, local, bP, ph, f,
-                                                     unused);
-  if (!local) scale_f<T, DIM>(ph, f, cprev, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr);
+  pull<T, S, DIM, !std::is_same<T, S>::value, false, LOCAL>(bF, bF, x, nb, m, eps, bP, ph, f,
+                                                            unused);
+  if constexpr (!LOCAL)
+    scale_f<T, DIM>(ph, f, cprev, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr);
   T p = clamp01(qsum<Q>(f));
   if (m & M_SOLID) p = T(0);
   return p;
 }
 
-template <typename T, typename S, int DIM, int BS>
+// LOCAL (first step after init_fields) is a kernel template parameter: the
+// host launches the matching instantiation, so no branch sits between the
+// geometry-byte load and the pulls.
+template <typename T, typename S, int DIM, int BS, bool LOCAL>
 __global__ __launch_bounds__(BS) void k_phi(Grid g, Phys<T> ph, const S* __restrict__ Fin,
                                             const T* __restrict__ phi_prev,
                                             T* __restrict__ phi_out,
                                             const uint8_t* __restrict__ meta,
                                             const T* __restrict__ eps,
                                             double* __restrict__ partials, const Ctrl* ctrl,
-                                            int slot_prev, int local) {
+                                            int slot_prev) {
   const long long N = g.N;
   const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t x = lb * BS + threadIdx.x;
   double v = 0.0;
   if (x < N) {
-    const T cf = local ? T(1) : (T)ctrl->c[slot_prev];
-    const T p = phi_node<T, S, DIM>(g, ph, Fin, phi_prev, meta, eps, cf, local != 0, x);
+    const T cf = LOCAL ? T(1) : (T)ctrl->c[slot_prev];
+    const T p = phi_node<T, S, DIM, LOCAL>(g, ph, Fin, phi_prev, meta, eps, cf, x);
     at(phi_out, x) = p;
     v = (double)p;
   }
@@ -667,7 +686,7 @@ template <typename T, typename S, int DIM> struct CollideArgs {
   int phiu_valid;
 };
 
-template <typename T, typename S, int DIM>
+template <typename T, typename S, int DIM, bool LOCAL>
 __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
                                              const Phys<T>& ph, uint32_t x) {
   constexpr int Q = Lat<DIM>::Q;
@@ -679,7 +698,6 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   const bool bnd = m & M_BOUNDARY;
   const Coord<DIM> c = decompose<DIM>(g, x);
   const Nbrs<DIM> nb = neighbours<DIM>(g, x, c);
-  const T eps = (m & M_PARTIAL) ? a.eps[x] : T(0);
 
   // g is pulled first and f only for the final f collision, after g* and u
   // are stored: f's 19 values are then never live together with g's, which
@@ -689,9 +707,9 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   const Buf bFi = dist_buf<DIM>(a.Fin, N), bGi = dist_buf<DIM>(a.Gin, N);
   const Buf bPp = field_buf(a.phi_prev, N), bPc = field_buf(a.phi_cur, N);
   const Buf bU = field_buf(a.u, N, DIM);
-  pull<T, S, DIM, false, false>(bGi, bGi, x, nb, m, eps, a.local != 0, bPp, ph, gg, unused);
+  pull<T, S, DIM, false, false, LOCAL>(bGi, bGi, x, nb, m, a.eps, bPp, ph, gg, unused);
   T pprev = T(0);
-  if (((m & M_TOP) && !a.local) || a.phiu_valid) pprev = a.phi_prev[x];
+  if (((m & M_TOP) && !LOCAL) || a.phiu_valid) pprev = a.phi_prev[x];
 
   // S1/S2: phi, rho, stencil, mu, correction
   const T p = at(a.phi_cur, x);
@@ -862,8 +880,8 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   asm volatile("" : "+v"(x2));
   const Nbrs<DIM> nb2 = neighbours<DIM>(g, x2, decompose<DIM>(g, x2));
   T f[Q];
-  pull<T, S, DIM, DEV, false>(bFi, bFi, x, nb2, m, eps, a.local != 0, bPp, ph, f, unused);
-  if (!a.local) scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot_prev], m, pprev, nullptr);
+  pull<T, S, DIM, DEV, false, LOCAL>(bFi, bFi, x, nb2, m, a.eps, bPp, ph, f, unused);
+  if constexpr (!LOCAL) scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot_prev], m, pprev, nullptr);
   T fsrc[Q];
   fsrc[0] = T(0);
 #pragma unroll
@@ -890,13 +908,13 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
 #define LBM_COLLIDE_ATTR
 #endif
 
-template <typename T, typename S, int DIM, int BS>
+template <typename T, typename S, int DIM, int BS, bool LOCAL>
 __global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_collide(CollideArgs<T, S, DIM> a,
                                                                  Phys<T> ph) {
   if (blockIdx.x == 0) finalize_mass<BS>(a.partials, a.nparts, a.ctrl_w, a.slot_new);
   const uint32_t x = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (x >= a.g.N) return;
-  collide_node<T, S, DIM>(a, ph, x);
+  collide_node<T, S, DIM, LOCAL>(a, ph, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -970,15 +988,17 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
     case OBS_P:
     case OBS_F:
     case OBS_G: {
-      const T e = (m & M_PARTIAL) ? a.eps[x] : T(0);
       T f[Q], unused[Q];
       const Buf bF = dist_buf<DIM>(a.Fsrc, N), bR = field_buf(a.pref, N);
       const Nbrs<DIM> nb = neighbours<DIM>(g, x, c);
-      if (a.kind == OBS_F)
-        pull<T, S, DIM, !std::is_same<T, S>::value, false>(bF, bF, x, nb, m, e, a.local != 0,
-                                                           bR, ph, f, unused);
-      else
-        pull<T, S, DIM, false, false>(bF, bF, x, nb, m, e, a.local != 0, bR, ph, f, unused);
+      constexpr bool DV = !std::is_same<T, S>::value;
+      if (a.kind == OBS_F) {
+        if (a.local) pull<T, S, DIM, DV, false, true>(bF, bF, x, nb, m, a.eps, bR, ph, f, unused);
+        else pull<T, S, DIM, DV, false, false>(bF, bF, x, nb, m, a.eps, bR, ph, f, unused);
+      } else {
+        if (a.local) pull<T, S, DIM, false, false, true>(bF, bF, x, nb, m, a.eps, bR, ph, f, unused);
+        else pull<T, S, DIM, false, false, false>(bF, bF, x, nb, m, a.eps, bR, ph, f, unused);
+      }
       if (a.kind == OBS_P) {
         a.out[ex] = qsum<Q>(f);
         break;
