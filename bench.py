@@ -81,6 +81,21 @@ def cpu_baseline(inputs, nsteps=2):
                       f"{platform.processor() or platform.machine()}"}
 
 
+def hbm_traffic(storage):
+    """Per-launch HBM bytes of pass B measured with rocprofv3 PMC counters
+    (FETCH_SIZE and WRITE_SIZE passes of this same command, each scaled by the
+    reading of a device copy of known size), as committed in
+    profiles/hbm_traffic.json; None when no measurement exists for `storage`."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)
+        t = rec["k_collide"][storage]
+        return int(t["bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,7 +104,8 @@ def main():
     ap.add_argument("--storage", default=None, choices=[None, "fp32", "bf16", "fp16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true", help="keep 256-thread blocks")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=6,
+                    help="steps of the numpy oracle timed as the CPU baseline (~1.2 s each)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a device copy of this many MiB (a known-size kernel "
                          "for scaling HBM byte counters)")
@@ -160,7 +176,8 @@ def main():
         "bytes_per_lu_step": b_step,
         "roofline": {"bound": "hbm", "kernel": "k_collide (pass B)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": hbm_traffic(args.storage or "fp32"),
                      "bytes_per_lu": b_coll, "avg_ms": round(tb, 5),
                      "block_sizes": list(bs)},
         "kernels_ms": {"k_phi": round(ta, 5), "k_collide": round(tb, 5),
