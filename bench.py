@@ -103,7 +103,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--storage", default=None, choices=[None, "fp32", "bf16", "fp16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-autotune", action="store_true", help="keep 256-thread blocks")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape (256-thread blocks, no occupancy cap)")
     ap.add_argument("--cpu-steps", type=int, default=6,
                     help="steps of the numpy oracle timed as the CPU baseline (~1.2 s each)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -131,9 +132,8 @@ def main():
     sixteen = args.storage in ("bf16", "fp16")
     b_step = B_STEP_16 if sixteen else B_STEP
     b_coll = B_COLLIDE_16 if sixteen else B_COLLIDE
-    bs = (256, 256)
     if not args.no_autotune:
-        bs = s.autotune(5)          # part of the warmup: 18 real steps
+        s.autotune(5)               # part of the warmup: 54 real steps
     s.step(args.warmup)
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -179,7 +179,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": hbm_traffic(args.storage or "fp32"),
                      "bytes_per_lu": b_coll, "avg_ms": round(tb, 5),
-                     "block_sizes": list(bs)},
+                     "launch": dict(zip(("block_phi", "block_collide", "cap_phi",
+                                         "cap_collide"), s.launch()))},
         "kernels_ms": {"k_phi": round(ta, 5), "k_collide": round(tb, 5),
                        "step_events": round(tstep, 5)},
         "mass_drift": abs(mass - mass0) / mass0,

@@ -141,9 +141,18 @@ int lbm_profile_steps(lbm_solver* s, int nsteps, void* stream, double* out);
  * (default 256).  Invalidates a captured graph. */
 int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide);
 
-/* Autotuner: times each candidate block size for pass A and pass B over
- * `nsteps_each` real steps (the solver advances by 3*(nsteps_each+1) steps)
- * and keeps the fastest of each; reports them in *best_phi / *best_collide.
+/* Occupancy caps of pass A and pass B in waves per SIMD (1..8; 0 = no cap),
+ * enforced with dynamic LDS per workgroup.  Invalidates a captured graph.
+ * No counterpart in the reference. */
+int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide);
+
+/* Current launch shape: out4 = {block size A, block size B, cap A, cap B}. */
+int lbm_get_launch(const lbm_solver* s, int* out4);
+
+/* Autotuner: times candidate (block size, occupancy cap) pairs for pass A and
+ * pass B over `nsteps_each` real steps each (the solver advances by
+ * 9*(nsteps_each+1) steps) and keeps the fastest of each; reports the block
+ * sizes in *best_phi / *best_collide (lbm_get_launch gives the caps).
  * Synchronises `stream`.  No counterpart in the reference. */
 int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,
                  int* best_collide);

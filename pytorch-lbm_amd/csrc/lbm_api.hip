@@ -39,6 +39,7 @@ static int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int BS = 256;
+constexpr int kTuneRounds = 9;   // autotuner rounds (steps: rounds * (n + 1))
 
 struct lbm_solver {
   lbm_params p;
@@ -79,6 +80,8 @@ struct lbm_solver {
   bool has_frac = false;
   int bsA = 256, bsB = 256;   // block sizes of pass A / pass B (autotuned)
   unsigned int nbA = 0, nbB = 0;
+  int capA = 0, capB = 0;     // occupancy caps, waves per SIMD (0: none)
+  unsigned int ldsA = 0, ldsB = 0;   // dynamic LDS per block enforcing the caps
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int gsteps = 0;
@@ -129,11 +132,11 @@ template <typename T, int DIM> static Phys<T> make_phys(const lbm_solver* s) {
   ph.rcp_cs2 = T(1) / ph.cs2;
   ph.rcp_two_cs2 = T(1) / ph.two_cs2;
   for (int i = 0; i < Q; ++i) {
-    ph.w[i] = (T)w32<DIM>(i);
-    const T woc = ph.w[i] / ph.cs2;                 // self._w / self.cs2
-    for (int d = 0; d < DIM; ++d) ph.gcoef[i][d] = woc * (T)E<DIM>(i, d);
-    ph.lcoef[i] = T(2) * woc;                        // 2.0 * wi
-    ph.linit[i] = T(2) * ph.w[i];                    // 2.0 * self._w[i]
+    const int c = wclass<DIM>(i);
+    ph.wc[c] = (T)w32<DIM>(i);
+    ph.woc[c] = ph.wc[c] / ph.cs2;                   // self._w / self.cs2
+    ph.lc[c] = T(2) * ph.woc[c];                     // 2.0 * wi
+    ph.lic[c] = T(2) * ph.wc[c];                     // 2.0 * self._w[i]
   }
   ph.four_beta = TT<T>(4.0 * p.beta);
   ph.kappa = TT<T>(p.kappa);
@@ -189,6 +192,25 @@ template <typename T, int DIM> static Phys<T> make_phys(const lbm_solver* s) {
 }
 
 static unsigned int nblk(long long n) { return (unsigned int)((n + BS - 1) / BS); }
+
+// Dynamic LDS per workgroup that caps residency at `cap` waves per SIMD for
+// blocks of `bs` threads (4 SIMDs and 160 KiB of LDS per CU; `stat` bytes of
+// static LDS per block).  The step kernels are memory bound, and more resident
+// waves than the memory system can keep in flight only add contention, so the
+// autotuner can trade occupancy for fewer concurrent streams.  0: no cap.
+static unsigned int lds_for_cap(int bs, int cap, unsigned int stat) {
+  if (cap <= 0) return 0;
+  const unsigned int cu = 160u * 1024u, gran = 512u;
+  const int wpb = bs / 64;
+  int bpc = cap * 4 / wpb;                 // blocks per CU allowed
+  if (bpc < 1) bpc = 1;
+  // smallest per-block total (static + dynamic, in LDS granules) that fits bpc
+  // blocks but not bpc + 1
+  unsigned int tot = cu / (unsigned int)(bpc + 1) + 1;
+  tot = (tot + gran - 1) / gran * gran;
+  if ((unsigned long long)tot * (unsigned int)bpc > cu) return 0;   // cannot express
+  return tot > stat ? tot - stat : 0;
+}
 
 // ---------------------------------------------------------------------------
 // Exported entry points: C linkage comes from the declarations in lbm_amd.h.
@@ -507,11 +529,11 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     const int pslot = (int)(s->k & 1), nslot = (int)((s->k + 1) & 1);
 #define LAUNCH_A(B)                                                                        \
   if (s->local)                                                                             \
-    k_phi<T, S, DIM, B, true><<<s->nbA, B, 0, st>>>(                                        \
+    k_phi<T, S, DIM, B, true><<<s->nbA, B, s->ldsA, st>>>(                                  \
         s->g, ph, (const S*)s->Fb[src], (const T*)s->phi[pslot], (T*)s->phi[nslot], s->meta, \
         (const T*)s->eps, s->partials, s->ctrl, pslot);                                      \
   else                                                                                      \
-    k_phi<T, S, DIM, B, false><<<s->nbA, B, 0, st>>>(                                       \
+    k_phi<T, S, DIM, B, false><<<s->nbA, B, s->ldsA, st>>>(                                 \
         s->g, ph, (const S*)s->Fb[src], (const T*)s->phi[pslot], (T*)s->phi[nslot], s->meta, \
         (const T*)s->eps, s->partials, s->ctrl, pslot)
     LBM_BS_SWITCH(s->bsA, LAUNCH_A)
@@ -540,8 +562,8 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     a.local = s->local ? 1 : 0;
     a.phiu_valid = s->phiu_valid ? 1 : 0;
 #define LAUNCH_B(B)                                               \
-  if (s->local) k_collide<T, S, DIM, B, true><<<s->nbB, B, 0, st>>>(a, ph); \
-  else k_collide<T, S, DIM, B, false><<<s->nbB, B, 0, st>>>(a, ph)
+  if (s->local) k_collide<T, S, DIM, B, true><<<s->nbB, B, s->ldsB, st>>>(a, ph); \
+  else k_collide<T, S, DIM, B, false><<<s->nbB, B, s->ldsB, st>>>(a, ph)
     LBM_BS_SWITCH(s->bsB, LAUNCH_B)
 #undef LAUNCH_B
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 2], st));
@@ -691,8 +713,25 @@ int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide) {
   s->bsB = bs_collide;
   s->nbA = (unsigned int)((s->N + bs_phi - 1) / bs_phi);
   s->nbB = (unsigned int)((s->N + bs_collide - 1) / bs_collide);
+  s->ldsA = lds_for_cap(s->bsA, s->capA, 0);
+  s->ldsB = lds_for_cap(s->bsB, s->capB, 1024);   // finalize_mass's 128 doubles
   if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
   if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+  return LBM_OK;
+}
+
+int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  if (cap_phi < 0 || cap_phi > 8 || cap_collide < 0 || cap_collide > 8)
+    return fail(LBM_EINVAL, "occupancy cap must be 0 (none) or 1..8 waves per SIMD");
+  s->capA = cap_phi;
+  s->capB = cap_collide;
+  return lbm_set_block_sizes(s, s->bsA, s->bsB);
+}
+
+int lbm_get_launch(const lbm_solver* s, int* out4) {
+  if (!s || !out4) return fail(LBM_EINVAL, "null argument");
+  out4[0] = s->bsA; out4[1] = s->bsB; out4[2] = s->capA; out4[3] = s->capB;
   return LBM_OK;
 }
 
@@ -701,26 +740,35 @@ int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,
   if (!s) return fail(LBM_EINVAL, "null solver");
   if (!s->inited) return fail(LBM_ESTATE, "autotune before init_fields()");
   if (nsteps_each < 2) nsteps_each = 2;
-  const int cand[3] = {128, 256, 512};
+  // (block size, occupancy cap) candidates; pass A and pass B are timed
+  // independently (events around each launch), so round k times candidate k
+  // of pass B together with candidate k % 6 of pass A
+  const int candB[kTuneRounds][2] = {{128, 0}, {256, 0}, {512, 0}, {128, 5}, {256, 5},
+                                     {512, 5}, {128, 4}, {256, 4}, {512, 4}};
+  const int candA[6][2] = {{128, 0}, {256, 0}, {512, 0}, {128, 6}, {256, 6}, {512, 6}};
   double bestA = 1e30, bestB = 1e30;
-  int bA = 256, bB = 256;
-  // pass A and pass B are timed independently (events around each launch), so
-  // one sweep over the candidates serves both
-  for (int c = 0; c < 3; ++c) {
-    int rc = lbm_set_block_sizes(s, cand[c], cand[c]);
+  int bA[2] = {256, 0}, bB[2] = {256, 0};
+  for (int c = 0; c < kTuneRounds; ++c) {
+    const int* a = candA[c % 6];
+    const int* b = candB[c];
+    s->capA = a[1];
+    s->capB = b[1];
+    int rc = lbm_set_block_sizes(s, a[0], b[0]);
     if (rc) return rc;
     double t[3];
     rc = lbm_profile_steps(s, 1, stream, t);   // first launch of this shape: untimed
     if (rc) return rc;
     rc = lbm_profile_steps(s, nsteps_each, stream, t);
     if (rc) return rc;
-    if (t[0] < bestA) { bestA = t[0]; bA = cand[c]; }
-    if (t[1] < bestB) { bestB = t[1]; bB = cand[c]; }
+    if (c < 6 && t[0] < bestA) { bestA = t[0]; bA[0] = a[0]; bA[1] = a[1]; }
+    if (t[1] < bestB) { bestB = t[1]; bB[0] = b[0]; bB[1] = b[1]; }
   }
-  int rc = lbm_set_block_sizes(s, bA, bB);
+  s->capA = bA[1];
+  s->capB = bB[1];
+  int rc = lbm_set_block_sizes(s, bA[0], bB[0]);
   if (rc) return rc;
-  if (best_phi) *best_phi = bA;
-  if (best_collide) *best_collide = bB;
+  if (best_phi) *best_phi = bA[0];
+  if (best_collide) *best_collide = bB[0];
   return LBM_OK;
 }
 

@@ -292,7 +292,7 @@ __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, con
     if constexpr (DEV) {
       const T r = bget<T, T>(pref, 0, 0, x);
 #pragma unroll
-      for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.w[i] * r;
+      for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.template w<DIM>(i) * r;
     }
     return;
   }
@@ -315,7 +315,7 @@ __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, con
     for (int i = 0; i < Q; ++i) {
       bf[i] = dload<Q, T, S>(F, OPP<DIM>(i), vx);
       if constexpr (WITH_G) bg[i] = dload<Q, T, S>(G, OPP<DIM>(i), vx);
-      if constexpr (DEV) bf[i] = bf[i] + ph.w[i] * bget<T, T>(pref, 0, 0, nb.i[OPP<DIM>(i)]);
+      if constexpr (DEV) bf[i] = bf[i] + ph.template w<DIM>(i) * bget<T, T>(pref, 0, 0, nb.i[OPP<DIM>(i)]);
     }
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -327,7 +327,7 @@ __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, con
   if constexpr (DEV) {
     const T r = bget<T, T>(pref, 0, 0, x);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.w[i] * r;
+    for (int i = 0; i < Q; ++i) f[i] = f[i] + ph.template w<DIM>(i) * r;
   }
   if (m & M_PARTIAL) {
     const T eps = eps_arr[x];
@@ -335,7 +335,7 @@ __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, con
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       T o = dload<Q, T, S>(F, OPP<DIM>(i), vx);
-      if constexpr (DEV) o = o + ph.w[i] * bget<T, T>(pref, 0, 0, nb.i[OPP<DIM>(i)]);
+      if constexpr (DEV) o = o + ph.template w<DIM>(i) * bget<T, T>(pref, 0, 0, nb.i[OPP<DIM>(i)]);
       f[i] = a * f[i] + eps * o;
       if constexpr (WITH_G) {
         const T og = dload<Q, T, S>(G, OPP<DIM>(i), vx);
@@ -449,15 +449,15 @@ __device__ __forceinline__ void ediv(const Phys<T>& ph, const T* v, T d, T r,
 
 // f_eq and g_eq (lbm/fe_ac_solver.py:346-367) from ec = (e.u)/cs2 and
 // usq_t = |u|^2 / (2 cs2).
-template <typename T>
+template <typename T, int DIM>
 __device__ __forceinline__ T feq_of(const Phys<T>& ph, int i, T phi, T ec, T usq_t) {
   const T sq = (ec * ec) / T(2);
-  return phi * (ph.w[i] * (((T(1) + ec) + sq) - usq_t));
+  return phi * (ph.template w<DIM>(i) * (((T(1) + ec) + sq) - usq_t));
 }
-template <typename T>
+template <typename T, int DIM>
 __device__ __forceinline__ T geq_of(const Phys<T>& ph, int i, T P, T ec, T usq_t) {
   const T sq = (ec * ec) / T(2);
-  return P * ph.w[i] + ph.w[i] * ((ec + sq) - usq_t);
+  return P * ph.template w<DIM>(i) + ph.template w<DIM>(i) * ((ec + sq) - usq_t);
 }
 
 template <typename T, typename S, int DIM, int BS>
@@ -487,9 +487,9 @@ __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __rest
     const uint32_t vx = dvoff<Q, S>(x);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      T fe = feq_of<T>(ph, i, p, ec[i], usq_t);
-      const T ge = geq_of<T>(ph, i, T(0), ec[i], usq_t);
-      if constexpr (DEV) fe = fe - ph.w[i] * p;
+      T fe = feq_of<T, DIM>(ph, i, p, ec[i], usq_t);
+      const T ge = geq_of<T, DIM>(ph, i, T(0), ec[i], usq_t);
+      if constexpr (DEV) fe = fe - ph.template w<DIM>(i) * p;
       dput<Q, T, S>(bF, i, vx, fe);
       dput<Q, T, S>(bG, i, vx, ge);
     }
@@ -608,10 +608,13 @@ __device__ __forceinline__ void stencil(const Phys<T>& ph, const Grid& g,
     if (!init_lap) {
 #pragma unroll
       for (int d = 0; d < DIM; ++d)
-        if (E<DIM>(i, d) != 0) grad[d] = grad[d] + ph.gcoef[i][d] * psi;
-      l = l + ph.lcoef[i] * diff;
+        if (E<DIM>(i, d) != 0) {
+          const T t = ph.woc[wclass<DIM>(i)] * psi;
+          grad[d] = E<DIM>(i, d) > 0 ? grad[d] + t : grad[d] - t;
+        }
+      l = l + ph.lc[wclass<DIM>(i)] * diff;
     } else {
-      l = l + (ph.linit[i] * diff) / ph.cs2;
+      l = l + (ph.lic[wclass<DIM>(i)] * diff) / ph.cs2;
     }
   }
   *lap = l;
@@ -751,7 +754,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
     for (int k = 0; k < NS; ++k) first[k] = true;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const T gn = gg[i] - geq_of<T>(ph, i, P, ec[i], usq_t);
+      const T gn = gg[i] - geq_of<T, DIM>(ph, i, P, ec[i], usq_t);
       int k = 0;
 #pragma unroll
       for (int aa = 0; aa < DIM; ++aa) {
@@ -857,13 +860,13 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
     G[0] = T(0);
 #pragma unroll
     for (int i = 1; i < Q; ++i)
-      if (OPP<DIM>(i) > i) G[i] = divk((ns * ph.w[i]) * edot<DIM>(i, F), rc, rrc);
+      if (OPP<DIM>(i) > i) G[i] = divk((ns * ph.template w<DIM>(i)) * edot<DIM>(i, F), rc, rrc);
 #pragma unroll
     for (int i = 1; i < Q; ++i)
       if (OPP<DIM>(i) < i) G[i] = -G[OPP<DIM>(i)];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const T ge = geq_of<T>(ph, i, P, ec[i], usq_n);
+      const T ge = geq_of<T, DIM>(ph, i, P, ec[i], usq_n);
       const T gs = (gg[i] - omg * (gg[i] - ge)) + G[i];
       dput<Q, T, S>(bGo, i, vx, gs);
     }
@@ -887,16 +890,16 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
 #pragma unroll
   for (int i = 1; i < Q; ++i)
     if (OPP<DIM>(i) > i)
-      fsrc[i] = divk((ph.acoef * ph.w[i]) * edot<DIM>(i, src), ph.cs2, ph.rcp_cs2);
+      fsrc[i] = divk((ph.acoef * ph.template w<DIM>(i)) * edot<DIM>(i, src), ph.cs2, ph.rcp_cs2);
 #pragma unroll
   for (int i = 1; i < Q; ++i)
     if (OPP<DIM>(i) < i) fsrc[i] = -fsrc[OPP<DIM>(i)];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const T fe = feq_of<T>(ph, i, p, ec[i], usq_n);
+    const T fe = feq_of<T, DIM>(ph, i, p, ec[i], usq_n);
     const T fs = (f[i] - ph.om_f * (f[i] - fe)) + fsrc[i];
     T fst = fs;
-    if constexpr (DEV) fst = fs - ph.w[i] * bget<T, T>(bPc, 0, 0, nb2.i[i]);
+    if constexpr (DEV) fst = fs - ph.template w<DIM>(i) * bget<T, T>(bPc, 0, 0, nb2.i[i]);
     dput<Q, T, S>(bFo, i, vx, fst);
   }
 }

@@ -74,6 +74,13 @@ template <int DIM> __host__ __device__ constexpr float W32(int i) {
   }
 }
 
+// Weight class of direction i: 0 rest, 1 axis, 2 diagonal (the lattice has
+// three distinct weights).
+template <int DIM> __host__ __device__ constexpr int wclass(int i) {
+  if constexpr (DIM == 3) return i == 0 ? 0 : (i <= 6 ? 1 : 2);
+  else return i == 0 ? 0 : (i <= 4 ? 1 : 2);
+}
+
 // Per-node geometry byte.
 enum : uint8_t {
   M_SOLID = 1u,

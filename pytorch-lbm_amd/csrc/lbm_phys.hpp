@@ -11,11 +11,15 @@ namespace lbm {
 
 enum WetMode : int { WET_NONE = 0, WET_LINEAR = 1, WET_QUADRATIC = 2 };
 
+// Direction-dependent constants are stored once per weight class (rest, axis,
+// diagonal): with 19 per-direction copies the kernels ran out of scalar
+// registers.  The stencil's (w_i / cs2) * e_id is +-woc[class] (e_id = +-1),
+// applied as an exact sign on the product.
 template <typename T> struct Phys {
-  T w[19];            // T(w32_i)
-  T gcoef[19][3];     // (w_i / cs2) * e_id      (lbm/fe_ac_solver.py:306,318)
-  T lcoef[19];        // 2 * (w_i / cs2)         (lbm/fe_ac_solver.py:319)
-  T linit[19];        // 2 * w_i                 (lbm/fe_ac_solver.py:293)
+  T wc[3];            // T(w32) per class
+  T woc[3];           // w / cs2                 (lbm/fe_ac_solver.py:306,318)
+  T lc[3];            // 2 * (w / cs2)           (lbm/fe_ac_solver.py:319)
+  T lic[3];           // 2 * w                   (lbm/fe_ac_solver.py:293)
   T cs2, two_cs2;
   T rcp_cs2, rcp_two_cs2, rcp_xi;   // RN(1/cs2), RN(1/(2 cs2)), RN(1/xi)
   T four_beta, kappa, rho_l, rho_g, rho_floor, drho, visc, xi, ac_scale;
@@ -37,6 +41,8 @@ template <typename T> struct Phys {
   int s12_on;
   T s12_r, s12_t;
   T thr_lo, thr_hi;   // 0.15, 0.85
+
+  template <int DIM> __host__ __device__ T w(int i) const { return wc[wclass<DIM>(i)]; }
 };
 
 }  // namespace lbm

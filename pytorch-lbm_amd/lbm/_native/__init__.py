@@ -89,6 +89,8 @@ _SIGS = {
                                           _vp]),
     "lbm_graph_launch": (ctypes.c_int, [_vp, _vp]),
     "lbm_set_block_sizes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "lbm_set_occupancy": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "lbm_get_launch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "lbm_case_footprint": (ctypes.c_size_t, [ctypes.POINTER(LbmParams)]),
     "lbm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "lbm_batch_destroy": (ctypes.c_int, [_vp]),

@@ -250,13 +250,14 @@ class AllenCahnSolver:
         return tuple(float(v) for v in out)
 
     def autotune(self, nsteps_each=10):
-        """Pick the fastest block sizes for the two step kernels on this case.
-        Advances the solver by 3*(nsteps_each+1) steps."""
+        """Pick the fastest launch shape (block size, occupancy cap) for the two
+        step kernels on this case.  Advances the solver by 9*(nsteps_each+1)
+        steps; returns the block sizes (launch() gives the caps too)."""
         a, b = ctypes.c_int(), ctypes.c_int()
         with torch.cuda.device(self.device):
             nat.check(nat.lib.lbm_autotune(self._h, int(nsteps_each), self._stream(),
                                            ctypes.byref(a), ctypes.byref(b)), "lbm_autotune")
-        n = 3 * (int(nsteps_each) + 1)
+        n = 9 * (max(int(nsteps_each), 2) + 1)
         self.step_count += n
         self.t += float(n)
         self._cache.clear()
@@ -266,6 +267,17 @@ class AllenCahnSolver:
     def set_block_sizes(self, bs_phi, bs_collide):
         nat.check(nat.lib.lbm_set_block_sizes(self._h, int(bs_phi), int(bs_collide)),
                   "lbm_set_block_sizes")
+
+    def set_occupancy(self, cap_phi, cap_collide):
+        """Cap resident waves per SIMD of the two step kernels (0: no cap)."""
+        nat.check(nat.lib.lbm_set_occupancy(self._h, int(cap_phi), int(cap_collide)),
+                  "lbm_set_occupancy")
+
+    def launch(self):
+        """(block size A, block size B, cap A, cap B) of the step kernels."""
+        out = (ctypes.c_int * 4)()
+        nat.check(nat.lib.lbm_get_launch(self._h, out), "lbm_get_launch")
+        return tuple(out)
 
     def capture_graph(self, nsteps=2):
         """Record `nsteps` (even) steps into a HIP graph owned by the solver; replay
