@@ -41,14 +41,14 @@ B_STEP_16 = 204
 B_COLLIDE_16 = 184
 
 
-def cfg2(storage=None, dtype="float32"):
+def cfg2(storage=None, dtype="float32", theta=162.0):
     import torch
     from geometry.substrate import create_substrate_with_fraction
     from lbm.fe_ac_solver import AllenCahnSolver
     from lbm.fe_config import FEConfig
     from lbm.fe_droplet import create_fe_droplet_with_impact
     cfg = FEConfig(nx=150, ny=150, nz=107, rho_l=1.0, rho_g=1.0 / 828.0, sigma=0.01424,
-                   xi=4.0, tau_l=0.53, tau_g=0.53, theta_eq=162.0, device="cuda:0")
+                   xi=4.0, tau_l=0.53, tau_g=0.53, theta_eq=theta, device="cuda:0")
     s = AllenCahnSolver(cfg, dtype=getattr(torch, dtype), geo_amplification=1.5,
                         geometric_wetting=True, storage=storage)
     solid, frac = create_substrate_with_fraction(150, 150, 107, "ridge", 1.0, 22.5)
@@ -96,6 +96,70 @@ def hbm_traffic(storage):
         return None
 
 
+def precision_report(steps=2000, thetas=(90.0, 120.0, 162.0), every=20):
+    """Storage forms against an fp64 run of the same kernels on the 150x150x107
+    ridge impact: relative L2 of phi and u over fluid nodes at steps
+    500/1000/1500/2000, the max spread factor D_max/D0 (D = max(Dx, Dy) of the
+    phi > 0.5 non-solid extents, sampled every `every` steps, D0 = 45) and the
+    contact-line radius about the impact axis (75, 75)."""
+    import torch
+    cps = [c for c in (500, 1000, 1500, 2000) if c <= steps]
+    forms = [("fp64", None, "float64"), ("fp32", None, "float32"),
+             ("bf16", "bf16", "float32"), ("fp16", "fp16", "float32")]
+    out = {}
+    worst = 0.0
+    for th in thetas:
+        ref = None
+        rows = {}
+        for name, storage, dt in forms:
+            s, inputs = cfg2(storage, dt, theta=th)
+            fluid = ~inputs[0]
+            snaps, dmax, rmax = {}, 0.0, 0.0
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            k = 0
+            while k < steps:
+                n = min(every, steps - k)
+                s.step(n)
+                k += n
+                m = s.measure()
+                dmax = max(dmax, max(m["Dx"], m["Dy"]))
+                r = s.contact_radius(75.0, 75.0)
+                rmax = max(rmax, r)
+                if k in cps:
+                    snaps[k] = (s.phi.double().cpu().numpy()[fluid],
+                                s.u.double().cpu().numpy()[:, fluid], r)
+            wall = time.perf_counter() - t0
+            rec = {"spread_factor_max": dmax / 45.0, "contact_radius_max": rmax,
+                   "contact_radius": {str(c): snaps[c][2] for c in cps},
+                   "nan": bool(m["nan"]), "wall_s": round(wall, 2)}
+            if name == "fp64":
+                ref = snaps, rec
+            else:
+                rs, rr = ref
+                rec["phi_l2"] = {str(c): float(np.linalg.norm(snaps[c][0] - rs[c][0]) /
+                                               np.linalg.norm(rs[c][0])) for c in cps}
+                rec["u_l2"] = {str(c): float(np.linalg.norm(snaps[c][1] - rs[c][1]) /
+                                             max(np.linalg.norm(rs[c][1]), 1e-300)) for c in cps}
+                e_sf = abs(rec["spread_factor_max"] / rr["spread_factor_max"] - 1.0)
+                e_rc = abs(rec["contact_radius_max"] - rr["contact_radius_max"]) / max(
+                    rr["contact_radius_max"], 1e-300)
+                rec["spread_factor_rel_err"] = e_sf
+                rec["contact_radius_rel_err"] = e_rc
+                worst = max(worst, e_sf, e_rc)
+            rows[name] = rec
+            print(f"[precision] theta {th:g} {name}: spread {rec['spread_factor_max']:.4f} "
+                  f"r_cl {rec['contact_radius_max']:.3f} ({wall:.1f} s)", file=sys.stderr, flush=True)
+            del s
+            torch.cuda.empty_cache()
+        out[f"{th:g}"] = rows
+    return {"metric": "precision forms vs an fp64 run of the same kernels (150x150x107 ridge "
+                      "impact, 2000 steps): worst relative error of max spread factor and "
+                      "contact-line radius",
+            "value": round(100 * worst, 4), "unit": "%", "higher_is_better": False,
+            "gate_pct": 1.0, "steps": steps, "n_gpus": 1, "thetas": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +171,9 @@ def main():
                     help="keep the default launch shape (256-thread blocks, no occupancy cap)")
     ap.add_argument("--cpu-steps", type=int, default=6,
                     help="steps of the numpy oracle timed as the CPU baseline (~1.2 s each)")
+    ap.add_argument("--mode", default="plain", choices=["plain", "precision"],
+                    help="plain: the roofline/MLUPS line (default); precision: storage forms "
+                         "against fp64 on the same case")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a device copy of this many MiB (a known-size kernel "
                          "for scaling HBM byte counters)")
@@ -117,6 +184,9 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.mode == "precision":
+        print(json.dumps(precision_report()), flush=True)
+        return
     print(f"[bench] building case on {torch.cuda.get_device_name(0)}", file=sys.stderr,
           flush=True)
     if args.copy_probe_mb:

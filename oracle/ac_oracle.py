@@ -661,3 +661,27 @@ def spread(phi, solid):
     out.update(Dx=Dx, Dy=Dy, Dz=Dz, k=Dx / Dy if Dy > 0 else 0.0,
                z_com=float(co[:, -1].mean()))
     return out
+
+
+def contact_radius(phi, solid, cx, cy):
+    """Contact-line radius: the largest horizontal distance from (cx, cy) of a
+    liquid node (phi > 0.5, not solid) on the first fluid layer above each
+    column's topmost solid node (the first-fluid-layer search of
+    lbm/fe_measure.py:208-218, taken per column so that curved substrates are
+    followed).  0 when no such node exists.  3D only."""
+    nx, ny, nz = phi.shape
+    has = solid.any(axis=2)
+    top = np.where(has, nz - 1 - np.argmax(solid[:, :, ::-1], axis=2), -1)
+    z = top + 1
+    ok = z < nz
+    zc = np.minimum(z, nz - 1)
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    p = phi[X, Y, zc]
+    s = solid[X, Y, zc]
+    liquid = ok & ~s & (p > 0.5)
+    if not liquid.any():
+        return 0.0
+    dx = X.astype(np.float64) - cx
+    dy = Y.astype(np.float64) - cy
+    r2 = dx * dx + dy * dy
+    return float(np.sqrt(r2[liquid].max()))

@@ -937,10 +937,11 @@ long long lbm_step_count(const lbm_solver* s) { return s ? s->k : -1; }
 template <typename T, int DIM>
 static int measure_impl(lbm_solver* s, double* out, hipStream_t st) {
   const Phys<T> ph = make_phys<T, DIM>(s);
-  k_measure<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(
+  const unsigned int nb = (unsigned int)((s->N + (long long)BS * kMeasPer - 1) / ((long long)BS * kMeasPer));
+  k_measure<T, DIM, BS><<<nb, BS, 0, st>>>(
       s->g, ph, (const T*)s->phi[s->k & 1], s->meta, s->ctrl, (int)(s->k & 1),
       s->k > 0 ? 1 : 0, (MeasRec*)s->mparts);
-  k_measure_final<DIM, BS><<<1, BS, 0, st>>>((const MeasRec*)s->mparts, s->nblocks, s->dout);
+  k_measure_final<DIM, BS><<<1, BS, 0, st>>>((const MeasRec*)s->mparts, nb, s->dout);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->dout, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

@@ -3,8 +3,8 @@
 // (phi > 0.5 and not solid), volume (sum of phi with solid set to 0), C_max,
 // z_com (mean last-axis coordinate of the liquid nodes), NaN flag; and the
 // contact-line radius on the first fluid layer above each column's surface.
-// Each block writes one partial record; the last block combines the records
-// in block order, so results do not depend on scheduling.
+// Each block writes one partial record; a one-block kernel combines the
+// records in a fixed order, so results do not depend on scheduling.
 #pragma once
 
 #include "lbm_kernels.hpp"
@@ -55,29 +55,38 @@ __device__ __forceinline__ void rec_empty(MeasRec& r) {
 }
 
 // Per-block records (block order is the reduction order of k_measure_final).
+// Thread t of block b folds nodes b*BS + t + k*(gridDim.x*BS), k = 0..kMeasPer-1,
+// so loads stay coalesced and the grid (a fixed function of N) is 16x smaller
+// than one node per thread.
+constexpr int kMeasPer = 16;
 template <typename T, int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_measure(Grid g, Phys<T> ph, const T* __restrict__ phi,
                                                 const uint8_t* __restrict__ meta,
                                                 const Ctrl* ctrl, int slot, int scaled,
                                                 MeasRec* __restrict__ parts) {
   __shared__ MeasRec s[BS];
-  const uint32_t x = blockIdx.x * BS + threadIdx.x;
   MeasRec r;
   rec_empty(r);
-  if (x < g.N) {
+  const T cf = scaled ? (T)ctrl->c[slot] : T(1);
+  const uint32_t stride = gridDim.x * BS;
+  for (int k = 0; k < kMeasPer; ++k) {
+    const uint32_t x = blockIdx.x * BS + threadIdx.x + (uint32_t)k * stride;
+    if (x >= g.N) break;
     const uint8_t m = meta[x];
-    const T cf = scaled ? (T)ctrl->c[slot] : T(1);
     T p = phi_observed<T, DIM>(ph, phi[x], m, cf, scaled);
     if (p != p) r.nan = 1;
     if (m & M_SOLID) p = T(0);
-    r.vol = (double)p;
-    r.cmax = (double)p;
+    r.vol += (double)p;
+    r.cmax = (double)p > r.cmax ? (double)p : r.cmax;
     if (p > T(0.5)) {
       const Coord<DIM> c = decompose<DIM>(g, x);
       const int cc[3] = {c.x, c.y, c.z};
-      for (int d = 0; d < DIM; ++d) { r.mn[d] = cc[d]; r.mx[d] = cc[d]; }
-      r.cnt = 1;
-      r.zsum = (unsigned long long)cc[DIM - 1];
+      for (int d = 0; d < DIM; ++d) {
+        r.mn[d] = cc[d] < r.mn[d] ? cc[d] : r.mn[d];
+        r.mx[d] = cc[d] > r.mx[d] ? cc[d] : r.mx[d];
+      }
+      r.cnt += 1;
+      r.zsum += (unsigned long long)cc[DIM - 1];
     }
   }
   s[threadIdx.x] = r;
@@ -86,34 +95,35 @@ __global__ __launch_bounds__(BS) void k_measure(Grid g, Phys<T> ph, const T* __r
   if (threadIdx.x == 0) parts[blockIdx.x] = s[0];
 }
 
-// One block folds the n block records in block order and writes
+// One block folds the n block records and writes
 // out[0..9] = Dx, Dy, Dz, k, volume, C_max, z_com, n_liquid, phi_sum, nan.
+// Thread t folds records t, t + BS, t + 2 BS, ... in that order, then the
+// block combines the BS partial records in a fixed tree: the result does not
+// depend on scheduling.
+__device__ __forceinline__ void fold_rec(MeasRec& a, const MeasRec& b) {
+  for (int d = 0; d < 3; ++d) {
+    a.mn[d] = b.mn[d] < a.mn[d] ? b.mn[d] : a.mn[d];
+    a.mx[d] = b.mx[d] > a.mx[d] ? b.mx[d] : a.mx[d];
+  }
+  a.cnt += b.cnt;
+  a.zsum += b.zsum;
+  a.vol += b.vol;
+  a.cmax = b.cmax > a.cmax ? b.cmax : a.cmax;
+  a.nan |= b.nan;
+}
+
 template <int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_measure_final(const MeasRec* __restrict__ parts,
                                                       unsigned int n, double* out) {
   __shared__ MeasRec s[BS];
   MeasRec acc;
   rec_empty(acc);
-  for (unsigned int base = 0; base < n; base += BS) {
-    MeasRec t;
-    if (base + threadIdx.x < n) t = parts[base + threadIdx.x];
-    else rec_empty(t);
-    s[threadIdx.x] = t;
-    __syncthreads();
-    combine_rec<BS>(s, BS);
-    if (threadIdx.x == 0) {
-      const MeasRec& b = s[0];
-      for (int d = 0; d < 3; ++d) {
-        acc.mn[d] = b.mn[d] < acc.mn[d] ? b.mn[d] : acc.mn[d];
-        acc.mx[d] = b.mx[d] > acc.mx[d] ? b.mx[d] : acc.mx[d];
-      }
-      acc.cnt += b.cnt; acc.zsum += b.zsum; acc.vol += b.vol;
-      acc.cmax = b.cmax > acc.cmax ? b.cmax : acc.cmax;
-      acc.nan |= b.nan;
-    }
-    __syncthreads();
-  }
+  for (unsigned int i = threadIdx.x; i < n; i += BS) fold_rec(acc, parts[i]);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  combine_rec<BS>(s, BS);
   if (threadIdx.x == 0) {
+    acc = s[0];
     double D[3] = {0, 0, 0};
     if (acc.cnt > 0)
       for (int d = 0; d < DIM; ++d) D[d] = (double)(acc.mx[d] - acc.mn[d] + 1);

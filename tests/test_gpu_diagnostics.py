@@ -1,0 +1,42 @@
+"""Device diagnostics (measure, contact_radius) against the oracle's
+brute-force-pinned restatements applied to the same phi."""
+import numpy as np
+import pytest
+import torch
+
+from _cases import config_kwargs, load, solver_kwargs
+from oracle.ac_oracle import contact_radius, spread
+
+pytestmark = pytest.mark.gpu
+
+
+def need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+@pytest.mark.parametrize("name", ["ridge162_readme_f32", "concave126_f32", "flat90_f32",
+                                  "flat162_2d_f32"])
+def test_measure_and_contact_radius(name):
+    need_gpu()
+    print(f"[gpu] diagnostics {name}", flush=True)
+    from lbm.fe_ac_solver import AllenCahnSolver
+    from lbm.fe_config import FEConfig
+    z, m = load(name)
+    cfg = FEConfig(**config_kwargs(m), device="cuda")
+    s = AllenCahnSolver(cfg, dtype=torch.float32, **solver_kwargs(m))
+    s.set_solid(z["solid"], z["fraction"] if "fraction" in z.files else None)
+    s.init_fields(z["phi_init"], z["u_init"])
+    for nsteps in (0, 7, 13):
+        s.step(nsteps)
+        phi = s.phi.cpu().numpy()
+        solid = z["solid"]
+        ref = spread(phi, solid)
+        got = s.measure()
+        for k in ("Dx", "Dy", "Dz", "C_max"):
+            assert got[k] == ref[k], (k, got[k], ref[k])
+        assert abs(got["volume"] - ref["volume"]) <= 1e-12 * max(1.0, ref["volume"])
+        assert abs(got["z_com"] - ref["z_com"]) <= 1e-12 * max(1.0, ref["z_com"])
+        if phi.ndim == 3:
+            cx, cy = phi.shape[0] / 2.0, phi.shape[1] / 2.0
+            assert s.contact_radius(cx, cy) == contact_radius(phi, solid, cx, cy)
