@@ -160,6 +160,118 @@ def precision_report(steps=2000, thetas=(90.0, 120.0, 162.0), every=20):
             "gate_pct": 1.0, "steps": steps, "n_gpus": 1, "thetas": out}
 
 
+def sweep_bench(steps=1500, every=50, storage="bf16", seed=2024, n=48):
+    """The 48-case uneven impact sweep: batched (all cases device-resident,
+    one launch per pass for all of them, graph-captured, autotuned launch
+    shape, `storage` distributions) against the same cases run one after
+    another through the plain fp32 single-case path (host generators,
+    set_solid, init_fields, step loop), both with on-device diagnostics every
+    `every` steps."""
+    import torch
+    from geometry.substrate import create_substrate_with_fraction
+    from lbm.batch import BatchEngine, sweep_cases
+    from lbm.fe_ac_solver import AllenCahnSolver
+    from lbm.fe_droplet import create_fe_droplet_with_impact
+    cases = sweep_cases(seed, n, steps)
+    lu = sum(c.nodes for c in cases) * steps
+
+    # batched
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng = BatchEngine(storage)
+    ids = [eng.add(c) for c in cases]
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    spread_b = {cid: 0.0 for cid in ids}
+    eng.step(1)
+    tune = eng.autotune(3)
+    done = eng.status(ids[0])[0]
+    g = every if every % 2 == 0 else every + 1
+    reps = (steps - done) // g
+    eng.capture_graph(g)
+    t_step0 = time.perf_counter()
+    for _ in range(reps):
+        eng.run_graph(1)
+        for cid in ids:
+            m = eng.measure(cid)
+            spread_b[cid] = max(spread_b[cid], max(m["Dx"], m["Dy"]))
+    rest = steps - eng.status(ids[0])[0]
+    if rest > 0:
+        eng.step(rest)
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    assert all(eng.status(cid) == (steps, steps) for cid in ids)
+    for cid in ids:
+        m = eng.measure(cid)
+        spread_b[cid] = max(spread_b[cid], max(m["Dx"], m["Dy"]))
+    t_b_step = time.perf_counter() - t_step0
+    del eng
+    torch.cuda.empty_cache()
+    print(f"[sweep] batched {storage}: {t_b:.2f} s (generation {t_gen:.2f} s)",
+          file=sys.stderr, flush=True)
+
+    # sequential plain fp32
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tts, spread_s, t_steps = [], [], 0.0
+    for c in cases:
+        nx, ny, nz = c.grid
+        cfg = c.config()
+        cfg.device = "cuda:0"
+        sol = AllenCahnSolver(cfg, dtype=torch.float32, geo_amplification=c.geo_amplification,
+                              geometric_wetting=c.geometric_wetting,
+                              boundary_relax=c.boundary_relax)
+        solid, frac = create_substrate_with_fraction(nx, ny, nz, c.substrate, c.R_star, c.R_drop)
+        sol.set_solid(solid, solid_fraction=frac)
+        has = solid.any(axis=2)
+        top = np.where(has, nz - 1 - np.argmax(solid[:, :, ::-1], axis=2), 0)
+        ridge_top = int(top.max()) if c.substrate == "concave" else int(top[nx // 2, ny // 2])
+        cz = min(ridge_top + 2.0 + c.R_drop, nz - c.R_drop - 2)
+        phi0, _, u0 = create_fe_droplet_with_impact(nx, ny, nz, center=(nx / 2.0, ny / 2.0, cz),
+                                                    radius=c.R_drop, xi=c.xi,
+                                                    u_impact=(0.0, 0.0, c.u_impact))
+        sol.init_fields(phi0, u0)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        best = 0.0
+        k = 0
+        while k < steps:
+            m_ = min(every, steps - k)
+            sol.step(m_)
+            k += m_
+            m = sol.measure()
+            best = max(best, max(m["Dx"], m["Dy"]))
+        torch.cuda.synchronize()
+        t_steps += time.perf_counter() - ts
+        spread_s.append(best)
+        tts.append(time.perf_counter() - t0)
+        del sol
+    t_s = time.perf_counter() - t0
+    torch.cuda.empty_cache()
+    print(f"[sweep] sequential fp32: {t_s:.2f} s (stepping {t_steps:.2f} s)",
+          file=sys.stderr, flush=True)
+    sf_err = max(abs(spread_b[cid] / spread_s[i] - 1.0) for i, cid in enumerate(ids))
+    return {
+        "metric": "sweep solves/hour, 48 uneven D3Q19 impact cases x 1500 steps (batched, "
+                  f"{storage} storage, graph-captured) vs the same cases run sequentially in "
+                  "the plain fp32 form",
+        "value": round(n / t_b * 3600.0, 1), "unit": "solves/h", "higher_is_better": True,
+        "n_gpus": 1, "steps": steps,
+        "speedup_vs_sequential": round(t_s / t_b, 3),
+        "batched": {"wall_s": round(t_b, 3), "generation_s": round(t_gen, 3),
+                    "stepping_s": round(t_b_step, 3), "MLUPS": round(lu / t_b / 1e6, 1),
+                    "storage": storage, "graph_steps": g, "block_sizes": list(tune)},
+        "sequential_fp32": {"wall_s": round(t_s, 3), "stepping_s": round(t_steps, 3),
+                            "MLUPS": round(lu / t_s / 1e6, 1),
+                            "solves_per_hour": round(n / t_s * 3600.0, 1),
+                            "p50_time_to_solution_s": round(float(np.percentile(tts, 50)), 3),
+                            "p99_time_to_solution_s": round(float(np.percentile(tts, 99)), 3)},
+        "stepping_speedup": round(t_steps / t_b_step, 3),
+        "spread_factor_max_rel_diff_batched_vs_sequential": round(sf_err, 5),
+        "cases": {"seed": seed, "n": n, "lattice_updates": lu},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,9 +283,10 @@ def main():
                     help="keep the default launch shape (256-thread blocks, no occupancy cap)")
     ap.add_argument("--cpu-steps", type=int, default=6,
                     help="steps of the numpy oracle timed as the CPU baseline (~1.2 s each)")
-    ap.add_argument("--mode", default="plain", choices=["plain", "precision"],
+    ap.add_argument("--mode", default="plain", choices=["plain", "precision", "sweep"],
                     help="plain: the roofline/MLUPS line (default); precision: storage forms "
-                         "against fp64 on the same case")
+                         "against fp64 on the same case; sweep: 48 uneven cases batched vs "
+                         "sequential")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a device copy of this many MiB (a known-size kernel "
                          "for scaling HBM byte counters)")
@@ -186,6 +299,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.mode == "precision":
         print(json.dumps(precision_report()), flush=True)
+        return
+    if args.mode == "sweep":
+        print(json.dumps(sweep_bench()), flush=True)
         return
     print(f"[bench] building case on {torch.cuda.get_device_name(0)}", file=sys.stderr,
           flush=True)

@@ -211,7 +211,8 @@ def sweep_cases(seed=2024, n=48, steps=1500):
     substrates cycling flat/ridge/convex/concave, R* in {0.5, 1, 2}, droplet
     radius 0.15 nx, u_z = -U[0.02, 0.08], theta in {90, 108, 126, 144, 162}.
     Convex R*=2 on the two larger grids puts the drop into the cap under the
-    placement rule, so those draws take R*=1 instead."""
+    placement rule (the drop is clamped below the lid), so R* is drawn again
+    for those cases until it is 0.5 or 1."""
     import numpy as np
     rng = np.random.default_rng(seed)
     grids = [(96, 96, 96), (128, 128, 96), (150, 150, 107)]
@@ -223,10 +224,17 @@ def sweep_cases(seed=2024, n=48, steps=1500):
         g = grids[int(rng.integers(0, 3))]
         sub = subs[i % 4]
         rs = rstars[int(rng.integers(0, 3))]
-        if sub == "convex" and rs == 2.0 and g[0] >= 128:
-            rs = 1.0
+        while sub == "convex" and rs == 2.0 and g[0] >= 128:
+            rs = rstars[int(rng.integers(0, 3))]
         th = thetas[int(rng.integers(0, 5))]
         u = -float(rng.uniform(0.02, 0.08))
         out.append(CaseSpec(grid=g, substrate=sub, R_star=rs, R_drop=0.15 * g[0], theta_eq=th,
                             u_impact=u, steps=steps, tag=f"c{i:02d}"))
     return out
+
+
+def case_table(cases):
+    """JSON-able rows of a case list (the committed sweep fixture format)."""
+    return [{"tag": c.tag, "grid": list(c.grid), "substrate": c.substrate, "R_star": c.R_star,
+             "R_drop": c.R_drop, "theta_eq": c.theta_eq, "u_impact": c.u_impact,
+             "steps": c.steps} for c in cases]

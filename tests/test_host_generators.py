@@ -51,3 +51,19 @@ def test_feconfig_closed_forms():
     assert c.dim == 3 and c.grid_shape == (150, 150, 107)
     c2 = FEConfig(nx=8, ny=8, nz=1, theta_eq=90.0, sigma=0.01, xi=4.0)
     assert c2.phi_c == 0.0 and c2.dim == 2
+
+
+def test_sweep_case_table_matches_fixture():
+    """The 48-case sweep drawn from seed 2024 is the committed case table."""
+    import json
+    import os
+    from lbm.batch import case_table, sweep_cases
+    here = os.path.dirname(os.path.abspath(__file__))
+    fx = json.load(open(os.path.join(here, "golden", "cfg4_cases.json")))
+    assert case_table(sweep_cases(fx["seed"], 48, 1500)) == fx["cases"]
+    rows = fx["cases"]
+    assert {tuple(r["grid"]) for r in rows} <= {(96, 96, 96), (128, 128, 96), (150, 150, 107)}
+    assert not any(r["substrate"] == "convex" and r["R_star"] == 2.0 and r["grid"][0] >= 128
+                   for r in rows)
+    assert all(abs(r["R_drop"] - 0.15 * r["grid"][0]) < 1e-12 for r in rows)
+    assert all(-0.08 <= r["u_impact"] <= -0.02 for r in rows)
