@@ -1,0 +1,156 @@
+"""Continuous-batching request loop for impact sweeps on one GPU.
+
+The reference runs sweep cases one process at a time from a queue file
+(scripts/night_queue.py:28-50, scripts/run_pending_sweep.py:207-279).  Here a
+single loop owns one BatchEngine: requests that arrive while others are on
+the GPU are admitted as soon as an execution slot and their HBM footprint
+are free, join the resident batch at the next chunk boundary, advance inside
+the same multi-case launches as everything else, and leave (measured, then
+evicted) when they reach their own step target; their memory and slot go to
+the next waiting request.  Lattices stay device-resident from admission to
+completion.
+
+Every case keeps its own buffers, reduction tree and step counter, so a
+request's result does not depend on what else shares the launches.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+from .batch import BatchEngine, CaseSpec, footprint
+
+SUBSTRATE_TYPES = ("flat", "ridge", "convex", "concave")
+
+
+@dataclass
+class Request:
+    spec: CaseSpec
+    t_arrival: float                 # seconds after the start of the run
+    t_admit: Optional[float] = None
+    t_done: Optional[float] = None
+    case_id: Optional[int] = None
+    result: Dict = field(default_factory=dict)
+
+    @property
+    def lattice_updates(self):
+        return self.spec.nodes * self.spec.steps
+
+
+def poisson_requests(seed=5, rate=4.0, duration=20.0, steps=(500, 3000),
+                     grid_xy=(64, 256), grid_z=(64, 160)):
+    """A seeded Poisson stream of impact solves: exponential inter-arrival
+    times of mean 1/rate over `duration` seconds; per request nx = ny ~
+    U{grid_xy}, nz ~ U{grid_z}, substrate ~ U(4 types), R* ~ U[0.5, 2],
+    theta ~ U[90, 162], |u_impact| ~ U[0.02, 0.08], steps ~ U{steps}; droplet
+    radius 0.15 min(nx, nz) (0.15 nx, the sweep's rule, would not fit the
+    lower domains), placed by the sweep driver's rule.  R* is capped where the
+    substrate top (R* R for ridge and cavity rim, 2 R* R for the convex
+    sphere) plus the 2-node gap and the drop would not fit under the lid."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out, t, i = [], 0.0, 0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > duration:
+            break
+        nxy = int(rng.integers(grid_xy[0], grid_xy[1] + 1))
+        nz = int(rng.integers(grid_z[0], grid_z[1] + 1))
+        sub = SUBSTRATE_TYPES[int(rng.integers(0, 4))]
+        rs = float(rng.uniform(0.5, 2.0))
+        th = float(rng.uniform(90.0, 162.0))
+        u = -float(rng.uniform(0.02, 0.08))
+        n = int(rng.integers(steps[0], steps[1] + 1))
+        r = 0.15 * min(nxy, nz)
+        room = nz - 2.0 * r - 4.0            # height left for the substrate
+        if sub in ("ridge", "concave"):
+            rs = min(rs, room / r)
+        elif sub == "convex":
+            rs = min(rs, room / (2.0 * r))
+        spec = CaseSpec(grid=(nxy, nxy, nz), substrate=sub, R_star=rs, R_drop=r,
+                        theta_eq=th, u_impact=u, steps=n, tag=f"q{i:03d}")
+        out.append(Request(spec=spec, t_arrival=t))
+        i += 1
+    return out
+
+
+class SweepServer:
+    """Admission by HBM footprint and execution slots; continuous batching.
+
+    budget_bytes: device bytes the resident cases may take (0: no limit);
+    max_active: execution slots (cases advanced together); chunk: steps per
+    batch launch sequence between admission/eviction points (shortened so
+    that no resident case overshoots its target)."""
+
+    def __init__(self, storage="bf16", budget_bytes=0, max_active=32, chunk=50,
+                 device=None):
+        self.engine = BatchEngine(storage, budget_bytes=budget_bytes, device=device)
+        self.storage = storage
+        self.budget = int(budget_bytes)
+        self.max_active = int(max_active)
+        self.chunk = int(chunk)
+        self.active: Dict[int, Request] = {}
+        self.done: List[Request] = []
+
+    def _fits(self, req):
+        if len(self.active) >= self.max_active:
+            return False
+        if self.budget <= 0:
+            return True
+        return self.engine.bytes_used() + footprint(req.spec, self.storage) <= self.budget
+
+    def _finish(self, req, now):
+        cid = req.case_id
+        m = self.engine.measure(cid)
+        req.result = {"Dx": m["Dx"], "Dy": m["Dy"], "Dz": m["Dz"], "z_com": m["z_com"],
+                      "volume": m["volume"], "nan": m["nan"],
+                      "contact_radius": self.engine.contact_radius(cid)}
+        if self.keep_fields:
+            req.result["phi"] = self.engine.field(cid, "phi").cpu()
+        req.t_done = now
+        self.engine.remove(cid)
+        del self.active[cid]
+        self.done.append(req)
+
+    def run(self, requests, keep_fields=False, on_event=None):
+        """Serve `requests` (sorted by arrival) in real time; returns them
+        with admission / completion times and results."""
+        import torch
+        self.keep_fields = keep_fields
+        pending = deque(sorted(requests, key=lambda r: r.t_arrival))
+        waiting: deque = deque()
+        t0 = time.perf_counter()
+        self.lattice_updates = 0
+        while pending or waiting or self.active:
+            now = time.perf_counter() - t0
+            while pending and pending[0].t_arrival <= now:
+                waiting.append(pending.popleft())
+            # FIFO admission: the head of the queue waits for room
+            while waiting and self._fits(waiting[0]):
+                req = waiting.popleft()
+                req.case_id = self.engine.add(req.spec)
+                req.t_admit = time.perf_counter() - t0
+                self.active[req.case_id] = req
+                if on_event:
+                    on_event("admit", req)
+            if not self.active:
+                if pending:
+                    time.sleep(max(0.0, min(0.05, pending[0].t_arrival - now)))
+                continue
+            left = min(r.spec.steps - self.engine.status(cid)[0]
+                       for cid, r in self.active.items())
+            n = max(1, min(self.chunk, left))
+            self.engine.step(n)
+            self.lattice_updates += n * sum(r.spec.nodes for r in self.active.values())
+            torch.cuda.current_stream(self.engine.device).synchronize()
+            now = time.perf_counter() - t0
+            for cid, req in list(self.active.items()):
+                k, target = self.engine.status(cid)
+                if k >= target:
+                    self._finish(req, now)
+                    if on_event:
+                        on_event("done", req)
+        self.wall = time.perf_counter() - t0
+        return self.done
