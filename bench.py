@@ -388,7 +388,7 @@ def main():
     b_step = B_STEP_16 if sixteen else B_STEP
     b_coll = B_COLLIDE_16 if sixteen else B_COLLIDE
     if not args.no_autotune:
-        s.autotune(5)               # part of the warmup: 54 real steps
+        s.autotune(5)               # part of the warmup: 108 real steps
     s.step(args.warmup)
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -435,7 +435,7 @@ def main():
                      "traffic": hbm_traffic(args.storage or "fp32"),
                      "bytes_per_lu": b_coll, "avg_ms": round(tb, 5),
                      "launch": dict(zip(("block_phi", "block_collide", "cap_phi",
-                                         "cap_collide"), s.launch()))},
+                                         "cap_collide", "split"), s.launch()))},
         "kernels_ms": {"k_phi": round(ta, 5), "k_collide": round(tb, 5),
                        "step_events": round(tstep, 5)},
         "mass_drift": abs(mass - mass0) / mass0,

@@ -146,13 +146,20 @@ int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide);
  * No counterpart in the reference. */
 int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide);
 
-/* Current launch shape: out4 = {block size A, block size B, cap A, cap B}. */
-int lbm_get_launch(const lbm_solver* s, int* out4);
+/* Pass B as two launches over 64-node chunk lists (1, default): plain-fluid
+ * chunks with a wall-free kernel (fewer registers, more resident waves), the
+ * other chunks with the general kernel; or one launch over all nodes (0).
+ * Results are identical either way.  No counterpart in the reference. */
+int lbm_set_split(lbm_solver* s, int split);
 
-/* Autotuner: times candidate (block size, occupancy cap) pairs for pass A and
- * pass B over `nsteps_each` real steps each (the solver advances by
- * 9*(nsteps_each+1) steps) and keeps the fastest of each; reports the block
- * sizes in *best_phi / *best_collide (lbm_get_launch gives the caps).
+/* Current launch shape: out5 = {block size A, block size B, cap A, cap B,
+ * split}. */
+int lbm_get_launch(const lbm_solver* s, int* out5);
+
+/* Autotuner: times candidate (block size, occupancy cap[, split]) shapes for
+ * pass A and pass B over `nsteps_each` real steps each (the solver advances
+ * by 18*(nsteps_each+1) steps) and keeps the fastest of each; reports the
+ * block sizes in *best_phi / *best_collide (lbm_get_launch gives the rest).
  * Synchronises `stream`.  No counterpart in the reference. */
 int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,
                  int* best_collide);
@@ -250,6 +257,10 @@ int lbm_batch_step(lbm_batch* b, int nsteps, void* stream);
 
 /* Block sizes of the batch kernels (128/256/512). */
 int lbm_batch_set_block_sizes(lbm_batch* b, int bs_phi, int bs_collide);
+
+/* Pass B of the batch as plain-fluid / other chunk-list launches (1, default)
+ * or one launch over all nodes (0); results are identical. */
+int lbm_batch_set_split(lbm_batch* b, int split);
 
 /* Time the 9 block-size pairs over nsteps_each steps each (the batch advances
  * by 9*(nsteps_each+1) steps) and keep the fastest.  Synchronises. */

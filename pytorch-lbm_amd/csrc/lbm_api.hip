@@ -39,7 +39,7 @@ static int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int BS = 256;
-constexpr int kTuneRounds = 9;   // autotuner rounds (steps: rounds * (n + 1))
+constexpr int kTuneRounds = 18;  // autotuner rounds (steps: rounds * (n + 1))
 
 struct lbm_solver {
   lbm_params p;
@@ -80,6 +80,12 @@ struct lbm_solver {
   bool has_frac = false;
   int bsA = 256, bsB = 256;   // block sizes of pass A / pass B (autotuned)
   unsigned int nbA = 0, nbB = 0;
+  // pass-B chunk lists (64-node chunks): plain-fluid chunks first, then the
+  // rest; built at set_geometry
+  uint32_t* chunks = nullptr;
+  uint8_t* chunk_flag = nullptr;
+  unsigned int n_plain = 0, n_wall = 0;
+  bool split = true;          // run pass B as the two chunk lists
   int capA = 0, capB = 0;     // occupancy caps, waves per SIMD (0: none)
   unsigned int ldsA = 0, ldsB = 0;   // dynamic LDS per block enforcing the caps
   hipGraphExec_t gexec = nullptr;
@@ -312,6 +318,8 @@ static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipS
   ALLOC(s->normal, D * N * 4);
   ALLOC(s->surf, (size_t)p.nx * p.ny * 2);
   ALLOC(s->partials, (size_t)((N + 63) / 64 + 1) * 8);   // room for 64-thread pass A blocks
+  ALLOC(s->chunks, (size_t)((N + 63) / 64) * 4);
+  ALLOC(s->chunk_flag, (size_t)((N + 63) / 64));
   ALLOC(s->mparts, (size_t)s->nblocks * sizeof(MeasRec));
   ALLOC(s->dout, 16 * 8);
   ALLOC(s->ctrl, sizeof(Ctrl));
@@ -332,7 +340,8 @@ int lbm_destroy(lbm_solver* s) {
   if (s->graph) (void)hipGraphDestroy(s->graph);
   void* ptrs[] = {s->Fb[0], s->Fb[1], s->Gb[0], s->Gb[1], s->u, s->phi[0], s->phi[1],
                   s->eps, s->ampf, s->meta, s->bnd, s->dist[0], s->dist[1], s->solid,
-                  s->wmask, s->normal, s->surf, s->partials, s->mparts, s->dout, s->ctrl};
+                  s->wmask, s->normal, s->surf, s->partials, s->mparts, s->dout, s->ctrl,
+                  s->chunks, s->chunk_flag};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete s;
@@ -369,7 +378,24 @@ static int set_geometry_impl(lbm_solver* s, const void* frac, int fdt, hipStream
                                                  (T*)s->eps, wet);
   const long long ncol = DIM == 3 ? (long long)g.nx * g.ny : g.nx;
   k_surface<DIM, BS><<<nblk(ncol), BS, 0, st>>>(g, s->solid, s->surf);
+  // pass-B chunk lists: plain-fluid chunks, then the others (set-up only: one
+  // read-back of a byte per 64 nodes)
+  const size_t nch = (size_t)((s->N + 63) / 64);
+  k_chunk_plain<BS><<<nblk((long long)nch * 64), BS, 0, st>>>(s->N, s->meta, s->chunk_flag);
   HIPCHK(hipGetLastError());
+  std::vector<uint8_t> flag(nch);
+  HIPCHK(hipMemcpyAsync(flag.data(), s->chunk_flag, nch, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<uint32_t> list;
+  list.reserve(nch);
+  for (size_t c = 0; c < nch; ++c)
+    if (flag[c]) list.push_back((uint32_t)c);
+  s->n_plain = (unsigned int)list.size();
+  for (size_t c = 0; c < nch; ++c)
+    if (!flag[c]) list.push_back((uint32_t)c);
+  s->n_wall = (unsigned int)(nch - s->n_plain);
+  HIPCHK(hipMemcpyAsync(s->chunks, list.data(), nch * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
   return LBM_OK;
 }
 
@@ -561,11 +587,32 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     a.slot_prev = pslot;
     a.local = s->local ? 1 : 0;
     a.phiu_valid = s->phiu_valid ? 1 : 0;
+    if (!s->split) {
 #define LAUNCH_B(B)                                               \
   if (s->local) k_collide<T, S, DIM, B, true><<<s->nbB, B, s->ldsB, st>>>(a, ph); \
   else k_collide<T, S, DIM, B, false><<<s->nbB, B, s->ldsB, st>>>(a, ph)
-    LBM_BS_SWITCH(s->bsB, LAUNCH_B)
+      LBM_BS_SWITCH(s->bsB, LAUNCH_B)
 #undef LAUNCH_B
+    } else {
+      // plain-fluid chunks, then the rest; the first launch forms the mass
+      // factor
+      const uint32_t* cp = s->chunks;
+      const uint32_t* cw = s->chunks + s->n_plain;
+      const unsigned int np = s->n_plain, nw = s->n_wall;
+#define LAUNCH_C(B)                                                                         \
+  {                                                                                         \
+    const unsigned int gp = (np + B / 64 - 1) / (B / 64), gw = (nw + B / 64 - 1) / (B / 64); \
+    if (s->local) {                                                                         \
+      if (np) k_collide_chunks<T, S, DIM, B, true, true><<<gp, B, s->ldsB, st>>>(a, ph, cp, np, 1); \
+      if (nw) k_collide_chunks<T, S, DIM, B, true, false><<<gw, B, s->ldsB, st>>>(a, ph, cw, nw, np == 0); \
+    } else {                                                                                \
+      if (np) k_collide_chunks<T, S, DIM, B, false, true><<<gp, B, s->ldsB, st>>>(a, ph, cp, np, 1); \
+      if (nw) k_collide_chunks<T, S, DIM, B, false, false><<<gw, B, s->ldsB, st>>>(a, ph, cw, nw, np == 0); \
+    }                                                                                       \
+  }
+      LBM_BS_SWITCH(s->bsB, LAUNCH_C)
+#undef LAUNCH_C
+    }
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 2], st));
     s->prev_local = s->local;
     s->local = false;
@@ -729,9 +776,16 @@ int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide) {
   return lbm_set_block_sizes(s, s->bsA, s->bsB);
 }
 
-int lbm_get_launch(const lbm_solver* s, int* out4) {
-  if (!s || !out4) return fail(LBM_EINVAL, "null argument");
-  out4[0] = s->bsA; out4[1] = s->bsB; out4[2] = s->capA; out4[3] = s->capB;
+int lbm_set_split(lbm_solver* s, int split) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  s->split = split != 0;
+  return lbm_set_block_sizes(s, s->bsA, s->bsB);
+}
+
+int lbm_get_launch(const lbm_solver* s, int* out5) {
+  if (!s || !out5) return fail(LBM_EINVAL, "null argument");
+  out5[0] = s->bsA; out5[1] = s->bsB; out5[2] = s->capA; out5[3] = s->capB;
+  out5[4] = s->split ? 1 : 0;
   return LBM_OK;
 }
 
@@ -740,19 +794,22 @@ int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,
   if (!s) return fail(LBM_EINVAL, "null solver");
   if (!s->inited) return fail(LBM_ESTATE, "autotune before init_fields()");
   if (nsteps_each < 2) nsteps_each = 2;
-  // (block size, occupancy cap) candidates; pass A and pass B are timed
-  // independently (events around each launch), so round k times candidate k
-  // of pass B together with candidate k % 6 of pass A
-  const int candB[kTuneRounds][2] = {{128, 0}, {256, 0}, {512, 0}, {128, 5}, {256, 5},
-                                     {512, 5}, {128, 4}, {256, 4}, {512, 4}};
+  // (block size, occupancy cap, chunk split) candidates; pass A and pass B
+  // are timed independently (events around each pass), so round k times
+  // candidate k of pass B together with candidate k % 6 of pass A
+  const int candB[kTuneRounds][3] = {
+      {128, 0, 0}, {256, 0, 0}, {512, 0, 0}, {128, 5, 0}, {256, 5, 0}, {512, 5, 0},
+      {128, 4, 0}, {256, 4, 0}, {512, 4, 0}, {128, 0, 1}, {256, 0, 1}, {512, 0, 1},
+      {128, 5, 1}, {256, 5, 1}, {512, 5, 1}, {128, 4, 1}, {256, 4, 1}, {512, 4, 1}};
   const int candA[6][2] = {{128, 0}, {256, 0}, {512, 0}, {128, 6}, {256, 6}, {512, 6}};
   double bestA = 1e30, bestB = 1e30;
-  int bA[2] = {256, 0}, bB[2] = {256, 0};
+  int bA[2] = {256, 0}, bB[3] = {256, 0, 1};
   for (int c = 0; c < kTuneRounds; ++c) {
     const int* a = candA[c % 6];
     const int* b = candB[c];
     s->capA = a[1];
     s->capB = b[1];
+    s->split = b[2] != 0;
     int rc = lbm_set_block_sizes(s, a[0], b[0]);
     if (rc) return rc;
     double t[3];
@@ -761,10 +818,11 @@ int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,
     rc = lbm_profile_steps(s, nsteps_each, stream, t);
     if (rc) return rc;
     if (c < 6 && t[0] < bestA) { bestA = t[0]; bA[0] = a[0]; bA[1] = a[1]; }
-    if (t[1] < bestB) { bestB = t[1]; bB[0] = b[0]; bB[1] = b[1]; }
+    if (t[1] < bestB) { bestB = t[1]; bB[0] = b[0]; bB[1] = b[1]; bB[2] = b[2]; }
   }
   s->capA = bA[1];
   s->capB = bB[1];
+  s->split = bB[2] != 0;
   int rc = lbm_set_block_sizes(s, bA[0], bB[0]);
   if (rc) return rc;
   if (best_phi) *best_phi = bA[0];
@@ -999,8 +1057,11 @@ struct lbm_batch {
   size_t dcases_cap = 0;
   uint32_t* mapA = nullptr;           // [2 * nblocksA]: case, base
   uint32_t* mapB = nullptr;
-  size_t mapA_cap = 0, mapB_cap = 0;
-  unsigned int nA = 0, nB = 0;        // grid sizes
+  uint32_t* mapP = nullptr;           // pass B over plain-fluid chunks
+  uint32_t* mapW = nullptr;           // pass B over the other chunks
+  size_t mapA_cap = 0, mapB_cap = 0, mapP_cap = 0, mapW_cap = 0;
+  unsigned int nA = 0, nB = 0, nP = 0, nW = 0;   // grid sizes
+  bool split = true;                  // pass B as the two chunk-list launches
   int ncase = 0;                      // descriptor count (== cs.size())
   // scratch for device-side case generation
   void* gen = nullptr;
@@ -1044,7 +1105,7 @@ int lbm_batch_destroy(lbm_batch* b) {
   batch_drop_graph(b);
   for (lbm_solver* s : b->cs)
     if (s) lbm_destroy(s);
-  void* ptrs[] = {b->dcases, b->mapA, b->mapB, b->gen};
+  void* ptrs[] = {b->dcases, b->mapA, b->mapB, b->mapP, b->mapW, b->gen};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete b;
@@ -1181,6 +1242,13 @@ int lbm_batch_status(lbm_batch* b, int id, long long* done, long long* target) {
 
 size_t lbm_batch_bytes_used(const lbm_batch* b) { return b ? b->used : 0; }
 
+int lbm_batch_set_split(lbm_batch* b, int split) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  b->split = split != 0;
+  b->dirty = true;
+  return LBM_OK;
+}
+
 int lbm_batch_set_block_sizes(lbm_batch* b, int bsA, int bsB) {
   if (!b) return fail(LBM_EINVAL, "null batch");
   auto ok = [](int x) { return x == 128 || x == 256 || x == 512; };
@@ -1195,7 +1263,7 @@ template <typename T, typename S>
 static int batch_upload(lbm_batch* b, hipStream_t st) {
   const int n = (int)b->cs.size();
   std::vector<BatchCase<T, S>> h((size_t)std::max(n, 1));
-  std::vector<uint32_t> mA, mB;
+  std::vector<uint32_t> mA, mB, mP, mW;
   for (int c = 0; c < n; ++c) {
     BatchCase<T, S>& d = h[c];
     std::memset(&d, 0, sizeof(d));
@@ -1220,6 +1288,12 @@ static int batch_upload(lbm_batch* b, hipStream_t st) {
     d.nbB = (unsigned int)((s->N + b->bsB - 1) / b->bsB);
     d.nbA8 = (d.nbA + kXCD - 1) / kXCD * kXCD;
     d.nbB8 = (d.nbB + kXCD - 1) / kXCD * kXCD;
+    d.chunks = s->chunks;
+    d.n_plain = s->n_plain;
+    d.n_wall = s->n_wall;
+    const unsigned int wpb = (unsigned int)(b->bsB / 64);
+    d.nbP8 = ((s->n_plain + wpb - 1) / wpb + kXCD - 1) / kXCD * kXCD;
+    d.nbW8 = ((s->n_wall + wpb - 1) / wpb + kXCD - 1) / kXCD * kXCD;
     d.k = s->k;
     d.k_end = b->k_end[c];
     d.local = s->local ? 1 : 0;
@@ -1228,6 +1302,11 @@ static int batch_upload(lbm_batch* b, hipStream_t st) {
     const uint32_t baseA = (uint32_t)(mA.size() / 2), baseB = (uint32_t)(mB.size() / 2);
     for (unsigned int i = 0; i < d.nbA8; ++i) { mA.push_back((uint32_t)c); mA.push_back(baseA); }
     for (unsigned int i = 0; i < d.nbB8; ++i) { mB.push_back((uint32_t)c); mB.push_back(baseB); }
+    const uint32_t baseP = (uint32_t)(mP.size() / 2), baseW = (uint32_t)(mW.size() / 2);
+    if (s->n_plain)
+      for (unsigned int i = 0; i < d.nbP8; ++i) { mP.push_back((uint32_t)c); mP.push_back(baseP); }
+    if (s->n_wall)
+      for (unsigned int i = 0; i < d.nbW8; ++i) { mW.push_back((uint32_t)c); mW.push_back(baseW); }
   }
   // split the interleaved (case, base) pairs into two arrays per pass
   auto upload_map = [&](std::vector<uint32_t>& m, uint32_t** dptr, size_t* cap,
@@ -1249,6 +1328,8 @@ static int batch_upload(lbm_batch* b, hipStream_t st) {
   };
   HIPCHK(upload_map(mA, &b->mapA, &b->mapA_cap, &b->nA));
   HIPCHK(upload_map(mB, &b->mapB, &b->mapB_cap, &b->nB));
+  HIPCHK(upload_map(mP, &b->mapP, &b->mapP_cap, &b->nP));
+  HIPCHK(upload_map(mW, &b->mapW, &b->mapW_cap, &b->nW));
   const size_t bytes = h.size() * sizeof(BatchCase<T, S>);
   if (bytes > b->dcases_cap) {
     if (b->dcases) { HIPCHK(hipStreamSynchronize(st)); (void)hipFree(b->dcases); }
@@ -1272,10 +1353,26 @@ static void batch_launch(lbm_batch* b, hipStream_t st) {
     LBM_BS_SWITCH(b->bsA, LA)
 #undef LA
   }
-  if (nB) {
+  if (!b->split) {
+    if (nB) {
 #define LB(B) k_collide_batch<T, S, DIM, B><<<nB, B, 0, st>>>(dc, mb)
-    LBM_BS_SWITCH(b->bsB, LB)
+      LBM_BS_SWITCH(b->bsB, LB)
 #undef LB
+    }
+  } else {
+    // plain-fluid chunks of every case, then the other chunks
+    const unsigned int nP = b->nP, nW = b->nW;
+    BlockMap mp{b->mapP, b->mapP + nP}, mw{b->mapW, b->mapW + nW};
+    if (nP) {
+#define LP(B) k_collide_batch_chunks<T, S, DIM, B, true><<<nP, B, 0, st>>>(dc, mp)
+      LBM_BS_SWITCH(b->bsB, LP)
+#undef LP
+    }
+    if (nW) {
+#define LW(B) k_collide_batch_chunks<T, S, DIM, B, false><<<nW, B, 0, st>>>(dc, mw)
+      LBM_BS_SWITCH(b->bsB, LW)
+#undef LW
+    }
   }
   k_advance_batch<T, S><<<(b->ncase + 63) / 64, 64, 0, st>>>((BatchCase<T, S>*)b->dcases,
                                                            b->ncase);

@@ -31,6 +31,9 @@ template <typename T, typename S> struct BatchCase {
   Ctrl* ctrl;
   unsigned int nbA, nbB;       // blocks of this case in pass A / pass B
   unsigned int nbA8, nbB8;     // rounded up to a multiple of the XCD count
+  const uint32_t* chunks;      // pass-B chunk lists: plain chunks, then the rest
+  unsigned int n_plain, n_wall;
+  unsigned int nbP8, nbW8;     // blocks of the two chunk lists (XCD multiples)
   long long k, k_end;          // steps done, step target
   int local, phiu_valid;
   int pad[2];
@@ -107,6 +110,55 @@ __global__ __launch_bounds__(BS) void k_collide_batch(const BatchCase<T, S>* __r
   a.phiu_valid = cs.phiu_valid;
   if (cs.local) collide_node<T, S, DIM, true>(a, cs.ph, x);
   else collide_node<T, S, DIM, false>(a, cs.ph, x);
+}
+
+// Pass B over the chunk lists of every case (one wave per 64-node chunk):
+// FLUID launches cover the plain-fluid chunks with the wall-free code, the
+// other launch the remaining chunks.  A case's mass factor is formed by its
+// block 0 in the first of the two launches that has blocks for it.
+template <typename T, typename S, int DIM, int BS, bool FLUID>
+__global__ __launch_bounds__(BS) void k_collide_batch_chunks(
+    const BatchCase<T, S>* __restrict__ cases, BlockMap map) {
+  const uint32_t ci = map.cs[blockIdx.x];
+  const BatchCase<T, S>& cs = cases[ci];
+  if (cs.k >= cs.k_end) return;
+  const unsigned int i = blockIdx.x - map.base[blockIdx.x];
+  const int pslot = (int)(cs.k & 1), nslot = (int)((cs.k + 1) & 1);
+  const long long N = cs.g.N;
+  if (i == 0 && (FLUID || cs.n_plain == 0))
+    finalize_mass<BS>(cs.partials, (unsigned int)((N + 63) / 64), cs.ctrl, nslot);
+  const unsigned int nch = FLUID ? cs.n_plain : cs.n_wall;
+  const unsigned int nb = (nch + BS / 64 - 1) / (BS / 64);
+  const unsigned int lb = xcd_block(i, FLUID ? cs.nbP8 : cs.nbW8);
+  if (lb >= nb) return;
+  const unsigned int w = lb * (BS / 64) + (threadIdx.x >> 6);
+  if (w >= nch) return;
+  const uint32_t x = (cs.chunks[(FLUID ? 0u : cs.n_plain) + w] << 6) + (threadIdx.x & 63u);
+  if (x >= N) return;
+  CollideArgs<T, S, DIM> a;
+  a.g = cs.g;
+  a.Fin = cs.F[pslot];
+  a.Gin = cs.G[pslot];
+  a.Fout = cs.F[nslot];
+  a.Gout = cs.G[nslot];
+  a.u = cs.u;
+  a.phi_cur = cs.phi[nslot];
+  a.phi_prev = cs.phi[pslot];
+  a.meta = cs.meta;
+  a.wmask = cs.wmask;
+  a.normal = cs.normal;
+  a.eps = cs.eps;
+  a.ampf = cs.ampf;
+  a.ctrl = cs.ctrl;
+  a.ctrl_w = cs.ctrl;
+  a.partials = cs.partials;
+  a.nparts = 0;
+  a.slot_prev = pslot;
+  a.slot_new = nslot;
+  a.local = cs.local;
+  a.phiu_valid = cs.phiu_valid;
+  if (cs.local) collide_node<T, S, DIM, true, FLUID>(a, cs.ph, x);
+  else collide_node<T, S, DIM, false, FLUID>(a, cs.ph, x);
 }
 
 // After pass B: every case that stepped advances its counter.

@@ -689,14 +689,16 @@ template <typename T, typename S, int DIM> struct CollideArgs {
   int phiu_valid;
 };
 
-template <typename T, typename S, int DIM, bool LOCAL>
+// FLUID: the caller guarantees a plain fluid node (no solid, partial, wall or
+// top flag), so every wall path compiles away.
+template <typename T, typename S, int DIM, bool LOCAL, bool FLUID = false>
 __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
                                              const Phys<T>& ph, uint32_t x) {
   constexpr int Q = Lat<DIM>::Q;
   constexpr bool DEV = !std::is_same<T, S>::value;
   const Grid& g = a.g;
   const long long N = g.N;
-  const uint8_t m = a.meta[x];
+  const uint8_t m = FLUID ? (uint8_t)(a.meta[x] & 0xF0u) : a.meta[x];
   const bool solid = m & M_SOLID;
   const bool bnd = m & M_BOUNDARY;
   const Coord<DIM> c = decompose<DIM>(g, x);
@@ -918,6 +920,35 @@ __global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_collide(CollideArgs<T, 
   const uint32_t x = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (x >= a.g.N) return;
   collide_node<T, S, DIM, LOCAL>(a, ph, x);
+}
+
+// Pass B over a list of 64-node chunks, one wave per chunk.  The chunks of a
+// grid are split once at set-up into plain ones (every node plain fluid),
+// run with FLUID = true -- no wall code, fewer registers, more resident
+// waves -- and the rest, run with the general code.  `finalize`: this launch
+// is the step's first pass-B launch and its block 0 forms the mass factor.
+template <typename T, typename S, int DIM, int BS, bool LOCAL, bool FLUID>
+__global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_collide_chunks(
+    CollideArgs<T, S, DIM> a, Phys<T> ph, const uint32_t* __restrict__ chunks,
+    unsigned int nchunks, int finalize) {
+  if (finalize && blockIdx.x == 0) finalize_mass<BS>(a.partials, a.nparts, a.ctrl_w, a.slot_new);
+  const unsigned int w = xcd_block(blockIdx.x, gridDim.x) * (BS / 64) + (threadIdx.x >> 6);
+  if (w >= nchunks) return;
+  const uint32_t x = (chunks[w] << 6) + (threadIdx.x & 63u);
+  if (x >= a.g.N) return;
+  collide_node<T, S, DIM, LOCAL, FLUID>(a, ph, x);
+}
+
+// Chunk classification: 1 when all 64 nodes of the chunk exist and are plain
+// fluid.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_chunk_plain(long long N, const uint8_t* __restrict__ meta,
+                                                   uint8_t* __restrict__ plain) {
+  const long long x = (long long)blockIdx.x * BS + threadIdx.x;
+  const bool ok = x < N && (meta[x] & 0x0Fu) == 0;
+  const unsigned long long all = __ballot(ok);
+  if ((threadIdx.x & 63) == 0 && (x >> 6) < (N + 63) / 64)
+    plain[x >> 6] = all == ~0ull ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -91,6 +91,7 @@ _SIGS = {
     "lbm_set_block_sizes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "lbm_set_occupancy": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "lbm_get_launch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "lbm_set_split": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbm_case_footprint": (ctypes.c_size_t, [ctypes.POINTER(LbmParams)]),
     "lbm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "lbm_batch_destroy": (ctypes.c_int, [_vp]),
@@ -103,6 +104,7 @@ _SIGS = {
     "lbm_batch_bytes_used": (ctypes.c_size_t, [_vp]),
     "lbm_batch_step": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "lbm_batch_set_block_sizes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "lbm_batch_set_split": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbm_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int)]),
     "lbm_batch_graph_capture": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),

@@ -148,6 +148,11 @@ class BatchEngine:
                       "lbm_batch_autotune")
         return a.value, b.value
 
+    def set_split(self, split):
+        """Pass B over plain-fluid / wall chunk lists (True, default) or all
+        nodes in one launch."""
+        nat.check(nat.lib.lbm_batch_set_split(self._h, 1 if split else 0), "lbm_batch_set_split")
+
     def capture_graph(self, nsteps):
         with torch.cuda.device(self.device):
             nat.check(nat.lib.lbm_batch_graph_capture(self._h, int(nsteps), self._stream()),

@@ -250,14 +250,15 @@ class AllenCahnSolver:
         return tuple(float(v) for v in out)
 
     def autotune(self, nsteps_each=10):
-        """Pick the fastest launch shape (block size, occupancy cap) for the two
-        step kernels on this case.  Advances the solver by 9*(nsteps_each+1)
-        steps; returns the block sizes (launch() gives the caps too)."""
+        """Pick the fastest launch shape (block size, occupancy cap, chunk split)
+        for the two step kernels on this case.  Advances the solver by
+        18*(nsteps_each+1) steps; returns the block sizes (launch() gives the
+        rest)."""
         a, b = ctypes.c_int(), ctypes.c_int()
         with torch.cuda.device(self.device):
             nat.check(nat.lib.lbm_autotune(self._h, int(nsteps_each), self._stream(),
                                            ctypes.byref(a), ctypes.byref(b)), "lbm_autotune")
-        n = 9 * (max(int(nsteps_each), 2) + 1)
+        n = 18 * (max(int(nsteps_each), 2) + 1)
         self.step_count += n
         self.t += float(n)
         self._cache.clear()
@@ -273,9 +274,13 @@ class AllenCahnSolver:
         nat.check(nat.lib.lbm_set_occupancy(self._h, int(cap_phi), int(cap_collide)),
                   "lbm_set_occupancy")
 
+    def set_split(self, split):
+        """Pass B over plain-fluid / wall chunk lists (True) or all nodes."""
+        nat.check(nat.lib.lbm_set_split(self._h, 1 if split else 0), "lbm_set_split")
+
     def launch(self):
-        """(block size A, block size B, cap A, cap B) of the step kernels."""
-        out = (ctypes.c_int * 4)()
+        """(block size A, block size B, cap A, cap B, split) of the step kernels."""
+        out = (ctypes.c_int * 5)()
         nat.check(nat.lib.lbm_get_launch(self._h, out), "lbm_get_launch")
         return tuple(out)
 

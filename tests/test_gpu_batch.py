@@ -43,8 +43,9 @@ def test_same_alone_and_in_the_mix(storage):
     mix.step(40)           # beyond every target: cases stop at their own
     for cid, s in zip(ids, sp):
         assert mix.status(cid) == (s.steps, s.steps)
-    for cid, s in zip(ids, sp):
+    for k, (cid, s) in enumerate(zip(ids, sp)):
         alone = BatchEngine(storage)
+        alone.set_split(k % 2 == 0)          # one launch per pass or chunk lists: same result
         a = alone.add(s)
         alone.step(s.steps)
         for name in ("phi", "u", "f"):

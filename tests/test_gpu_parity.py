@@ -216,12 +216,13 @@ def test_block_sizes_do_not_change_results():
     print("[gpu] block-size invariance", flush=True)
     ref, z, m = gpu_solver("ridge162_readme_f32", torch.float32)
     ref.step(12)
-    for bsa, bsb, ca, cb in ((128, 512, 0, 0), (512, 128, 0, 0), (256, 256, 6, 4),
-                             (128, 256, 6, 5)):
+    for bsa, bsb, ca, cb, sp in ((128, 512, 0, 0, 0), (512, 128, 0, 0, 1), (256, 256, 6, 4, 0),
+                                 (128, 256, 6, 5, 1)):
         s, _, _ = gpu_solver("ridge162_readme_f32", torch.float32)
         s.set_block_sizes(bsa, bsb)
         s.set_occupancy(ca, cb)
-        assert s.launch() == (bsa, bsb, ca, cb)
+        s.set_split(sp)
+        assert s.launch() == (bsa, bsb, ca, cb, sp)
         s.step(12)
         for key in ("phi", "u", "f", "g"):
             np.testing.assert_array_equal(np_(getattr(s, key)), np_(getattr(ref, key)))
@@ -232,6 +233,6 @@ def test_autotune_picks_valid_sizes():
     s, _, _ = gpu_solver("ridge120_f32", torch.float32)
     a, b = s.autotune(3)
     assert a in (128, 256, 512) and b in (128, 256, 512)
-    la, lb, ca, cb = s.launch()
-    assert (la, lb) == (a, b) and ca in (0, 6) and cb in (0, 4, 5)
-    assert s.step_count == 36
+    la, lb, ca, cb, sp = s.launch()
+    assert (la, lb) == (a, b) and ca in (0, 6) and cb in (0, 4, 5) and sp in (0, 1)
+    assert s.step_count == 72
