@@ -52,7 +52,7 @@ __global__ __launch_bounds__(BS) void k_phi_batch(const BatchCase<T, S>* __restr
   const BatchCase<T, S>& cs = cases[ci];
   if (cs.k >= cs.k_end) return;
   const unsigned int i = blockIdx.x - map.base[blockIdx.x];
-  const unsigned int lb = xcd_block(i, cs.nbA8);
+  const unsigned int lb = xcd_block_rev(i, cs.nbA8);   // pass B's last writes first
   if (lb >= cs.nbA) return;
   const long long N = cs.g.N;
   const uint32_t x = lb * BS + threadIdx.x;

@@ -242,6 +242,13 @@ __device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int n
   const unsigned int chunk = nb / kXCD, rem = nb % kXCD;
   return j * chunk + (j < rem ? j : rem) + i;
 }
+// The same ranges, each walked from its end.
+__device__ __forceinline__ unsigned int xcd_block_rev(unsigned int b, unsigned int nb) {
+  const unsigned int j = b % kXCD, i = b / kXCD;
+  const unsigned int chunk = nb / kXCD, rem = nb % kXCD;
+  const unsigned int nj = chunk + (j < rem ? 1u : 0u);
+  return j * chunk + (j < rem ? j : rem) + (nj - 1u - i);
+}
 
 // ---------------------------------------------------------------------------
 // Neighbour indices of one node: i[k] = index of x + e_k (periodic), i[0] = x.
@@ -543,7 +550,10 @@ __global__ __launch_bounds__(BS) void k_phi(Grid g, Phys<T> ph, const S* __restr
                                             double* __restrict__ partials, const Ctrl* ctrl,
                                             int slot_prev) {
   const long long N = g.N;
-  const unsigned int lb = xcd_block(blockIdx.x, gridDim.x);
+  // each XCD walks its range from the end: the distributions pass B wrote
+  // last are still in the memory-side cache and are read first (measured on
+  // the 150x150x107 case: 50.5 -> 44.5 us)
+  const unsigned int lb = xcd_block_rev(blockIdx.x, gridDim.x);
   const uint32_t x = lb * BS + threadIdx.x;
   double v = 0.0;
   if (x < N) {
