@@ -187,6 +187,18 @@ int lbm_set_mass_target(lbm_solver* s, double m0);
 /* Number of completed steps (solver.step_count). */
 long long lbm_step_count(const lbm_solver* s);
 
+/* Solver state snapshot for preemption and checkpoints (no counterpart in
+ * the reference, which has no solver checkpoint: its sweep resume only skips
+ * finished cases, scripts/run_pending_sweep.py:207-240).  The image holds
+ * everything step() reads and the observables need -- distributions, velocity,
+ * both phi buffers, mass factors, step count -- but not the geometry: restore
+ * into a solver of the same grid and precision after the same
+ * lbm_set_geometry.  `dst` / `src` may be device or host memory of
+ * lbm_state_bytes() bytes.  Both synchronise `stream`. */
+size_t lbm_state_bytes(const lbm_solver* s);
+int lbm_save_state(lbm_solver* s, void* dst, void* stream);
+int lbm_load_state(lbm_solver* s, const void* src, void* stream);
+
 /* Diagnostics of lbm/fe_measure.py:11-82 and scripts/run_thesis_sweep.py:
  * 185-206 on the current phi, reduced on the device:
  * out[0..9] = Dx, Dy, Dz, k, volume, C_max, z_com, n_liquid, phi_sum, nan_flag.

@@ -991,6 +991,105 @@ int lbm_set_mass_target(lbm_solver* s, double m0) {
 
 long long lbm_step_count(const lbm_solver* s) { return s ? s->k : -1; }
 
+// ---------------------------------------------------------------------------
+// Solver state snapshot (preemption / checkpoint).  Layout of the byte image:
+//   [0, 256)   StateHeader
+//   then       F[cur], G[cur], G[cur^1] (AoSoA, the step's distributions and
+//              the previous g set the P observable is pulled from), u,
+//              phi[0], phi[1], Ctrl
+// each array starting at a 256-byte boundary.  Geometry is not included: the
+// restoring solver gets the same set_geometry first.
+// ---------------------------------------------------------------------------
+namespace {
+struct StateHeader {
+  uint32_t magic, version;
+  int32_t nx, ny, nz, precision;
+  long long k;
+  int32_t cur, local, prev_local, phiu_valid;
+  uint64_t bytes;
+};
+constexpr uint32_t kStateMagic = 0x4c424d53u;   // "LBMS"
+constexpr uint32_t kStateVersion = 1;
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+struct StateLayout {
+  size_t dist, u, phi, off[8], total;
+};
+
+StateLayout state_layout(const lbm_solver* s) {
+  StateLayout L;
+  L.dist = (size_t)s->Q * (size_t)padded_nodes(s->N) * s->dsz;
+  L.u = (size_t)s->dim * (size_t)s->N * s->esz;
+  L.phi = (size_t)s->N * s->esz;
+  const size_t sz[7] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl)};
+  size_t o = 256;
+  for (int i = 0; i < 7; ++i) {
+    L.off[i] = o;
+    o += align256(sz[i]);
+  }
+  L.total = o;
+  return L;
+}
+}  // namespace
+
+size_t lbm_state_bytes(const lbm_solver* s) { return s ? state_layout(s).total : 0; }
+
+int lbm_save_state(lbm_solver* s, void* dst, void* stream) {
+  if (!s || !dst) return fail(LBM_EINVAL, "null argument");
+  if (!s->inited) return fail(LBM_ESTATE, "nothing to save before init_fields()");
+  hipStream_t st = (hipStream_t)stream;
+  const StateLayout L = state_layout(s);
+  StateHeader h{};
+  h.magic = kStateMagic;
+  h.version = kStateVersion;
+  h.nx = s->p.nx; h.ny = s->p.ny; h.nz = s->p.nz; h.precision = s->p.precision;
+  h.k = s->k;
+  h.cur = s->cur; h.local = s->local; h.prev_local = s->prev_local; h.phiu_valid = s->phiu_valid;
+  h.bytes = L.total;
+  char* d = (char*)dst;
+  const void* src[7] = {s->Fb[s->cur], s->Gb[s->cur], s->Gb[s->cur ^ 1], s->u, s->phi[0],
+                        s->phi[1], s->ctrl};
+  const size_t sz[7] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl)};
+  for (int i = 0; i < 7; ++i)
+    HIPCHK(hipMemcpyAsync(d + L.off[i], src[i], sz[i], hipMemcpyDefault, st));
+  // the header comes from host memory that goes out of scope: synchronous copy
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(d, &h, sizeof(h), hipMemcpyDefault));
+  return LBM_OK;
+}
+
+int lbm_load_state(lbm_solver* s, const void* src, void* stream) {
+  if (!s || !src) return fail(LBM_EINVAL, "null argument");
+  if (!s->geom) return fail(LBM_ESTATE, "set_geometry before load_state");
+  hipStream_t st = (hipStream_t)stream;
+  StateHeader h{};
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(&h, src, sizeof(h), hipMemcpyDefault));
+  const StateLayout L = state_layout(s);
+  if (h.magic != kStateMagic || h.version != kStateVersion)
+    return fail(LBM_EINVAL, "not a solver state image");
+  if (h.nx != s->p.nx || h.ny != s->p.ny || h.nz != s->p.nz || h.precision != s->p.precision ||
+      h.bytes != L.total)
+    return fail(LBM_EINVAL, "state image is for another grid or precision");
+  if (h.cur < 0 || h.cur > 1) return fail(LBM_EINVAL, "corrupt state image");
+  const char* d = (const char*)src;
+  void* dstp[7] = {s->Fb[h.cur], s->Gb[h.cur], s->Gb[h.cur ^ 1], s->u, s->phi[0], s->phi[1],
+                   s->ctrl};
+  const size_t sz[7] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl)};
+  for (int i = 0; i < 7; ++i)
+    HIPCHK(hipMemcpyAsync(dstp[i], d + L.off[i], sz[i], hipMemcpyDefault, st));
+  s->k = h.k;
+  s->cur = h.cur;
+  s->local = h.local != 0;
+  s->prev_local = h.prev_local != 0;
+  s->phiu_valid = h.phiu_valid != 0;
+  s->inited = true;
+  if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+  if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+  return LBM_OK;
+}
+
 
 template <typename T, int DIM>
 static int measure_impl(lbm_solver* s, double* out, hipStream_t st) {

@@ -92,6 +92,9 @@ _SIGS = {
     "lbm_set_occupancy": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "lbm_get_launch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "lbm_set_split": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbm_state_bytes": (ctypes.c_size_t, [_vp]),
+    "lbm_save_state": (ctypes.c_int, [_vp, _vp, _vp]),
+    "lbm_load_state": (ctypes.c_int, [_vp, _vp, _vp]),
     "lbm_case_footprint": (ctypes.c_size_t, [ctypes.POINTER(LbmParams)]),
     "lbm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "lbm_batch_destroy": (ctypes.c_int, [_vp]),

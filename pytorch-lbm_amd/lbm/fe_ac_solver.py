@@ -274,6 +274,33 @@ class AllenCahnSolver:
         nat.check(nat.lib.lbm_set_occupancy(self._h, int(cap_phi), int(cap_collide)),
                   "lbm_set_occupancy")
 
+    def state_dict(self, device=None):
+        """A snapshot of the solver state as a uint8 tensor (on the solver's
+        device unless `device` says otherwise): distributions, velocity, phi
+        buffers, mass factors, step count.  Restore with load_state_dict on a
+        solver with the same grid, precision and geometry."""
+        n = int(nat.lib.lbm_state_bytes(self._h))
+        t = torch.empty(n, dtype=torch.uint8, device=device or self.device)
+        with torch.cuda.device(self.device):
+            nat.check(nat.lib.lbm_save_state(self._h, nat.ptr(t), self._stream()),
+                      "lbm_save_state")
+        return t
+
+    def load_state_dict(self, state):
+        """Restore a snapshot taken by state_dict()."""
+        t = state.contiguous()
+        with torch.cuda.device(self.device):
+            nat.check(nat.lib.lbm_load_state(self._h, nat.ptr(t), self._stream()),
+                      "lbm_load_state")
+        self.step_count = int(nat.lib.lbm_step_count(self._h))
+        self.t = float(self.step_count)
+        self._cache.clear()
+        self._overrides.clear()
+        m0 = ctypes.c_double()
+        nat.check(nat.lib.lbm_get_mass(self._h, ctypes.byref(m0), None, self._stream()),
+                  "lbm_get_mass")
+        self.phi_mass_init = m0.value
+
     def set_split(self, split):
         """Pass B over plain-fluid / wall chunk lists (True) or all nodes."""
         nat.check(nat.lib.lbm_set_split(self._h, 1 if split else 0), "lbm_set_split")
