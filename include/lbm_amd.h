@@ -187,6 +187,19 @@ int lbm_set_mass_target(lbm_solver* s, double m0);
 /* Number of completed steps (solver.step_count). */
 long long lbm_step_count(const lbm_solver* s);
 
+/* Mid-run droplet injection, AllenCahnSolver.inject_droplet
+ * (lbm/fe_ac_solver.py:645-732), between two steps.  `c_dev` is the new
+ * drop's indicator C in [0, 1] (N elements of the compute dtype, public
+ * order; the reference builds it as 0.5 + 0.5 tanh(2 (R - |x - c|) / xi),
+ * clipped, in float32); `u_impact` holds `dim` doubles (NULL: at rest).
+ * Where C > 0.1: phi = max(phi, C) and f, g = f_eq(phi, u), g_eq(P, u);
+ * everywhere u = (1 - C) u + C u_impact; solid nodes keep phi = u = 0; the
+ * mass target becomes the fp64 sum of the new phi and the next step has no
+ * d(phi u) term.  Until the next step, LBM_FIELD_P and LBM_FIELD_MU report
+ * the values of the last step, as the reference keeps them.  Drops a captured
+ * graph.  Not for cases of a batch engine.  Asynchronous on `stream`. */
+int lbm_inject_droplet(lbm_solver* s, const void* c_dev, const double* u_impact, void* stream);
+
 /* Solver state snapshot for preemption and checkpoints (no counterpart in
  * the reference, which has no solver checkpoint: its sweep resume only skips
  * finished cases, scripts/run_pending_sweep.py:207-240).  The image holds

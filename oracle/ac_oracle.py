@@ -483,6 +483,53 @@ class OracleSolver:
             self.g = round16(self.g, self.storage)
         self.phi_u_prev = None
 
+    def inject_droplet(self, center, radius, xi, u_impact=None):
+        """Mid-run droplet injection (lbm/fe_ac_solver.py:645-732): where the
+        new drop's C > 0.1, phi = max(phi, C); u blends towards u_impact with
+        weight C everywhere; f, g re-equilibrate at those core nodes (g with
+        the current P); solid nodes keep phi = 0 and u = 0; M0 becomes the new
+        sum of phi (fp64 here) and the next step has no d(phi u).  With 16-bit
+        storage the whole state is stored again in its deviation form."""
+        dt = self.dt
+        shape = self.phi.shape
+        D = len(shape)
+        if u_impact is None:
+            u_impact = (0.0,) * D
+        mesh = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+        dist_sq = 0
+        for d, M in enumerate(mesh):
+            dist_sq = dist_sq + (M - center[d]) ** 2
+        dist = np.sqrt(dist_sq)
+        C = np.clip(0.5 + 0.5 * np.tanh(2.0 * (radius - dist) / xi), 0.0, 1.0)
+        C = C.astype(np.float32).astype(dt)
+        core = C > 0.1
+        solid = self.solid
+        phi = np.where(core, np.maximum(self.phi, C), self.phi)
+        phi = np.clip(phi, 0.0, 1.0).astype(dt)
+        phi = np.where(solid, dt.type(0.0), phi)
+        u_new = np.zeros((D,) + shape, dtype=dt)
+        for d in range(D):
+            u_new[d] = u_impact[d]
+        Cw = C[None]
+        u = ((1.0 - Cw) * self.u + Cw * u_new).astype(dt)
+        u = np.where(solid[None], dt.type(0.0), u)
+        feq = self.f_eq(phi, u)
+        geq = self.g_eq(self.P, u)
+        f = np.where(core[None], feq, self.f).astype(dt)
+        g = np.where(core[None], geq, self.g).astype(dt)
+        if self.storage is not None:
+            wv = self._wv()
+            f = round16(f - wv * phi[None], self.storage) + wv * phi[None]
+            g = round16(g, self.storage)
+        self.phi = phi
+        self.rho = (phi * self.p.rho_l + (1.0 - phi) * self.p.rho_g).astype(dt)
+        self.u = u
+        self.f = f
+        self.g = g
+        self.M0 = float(np.sum(phi, dtype=np.float64))
+        self.phi_u_prev = None
+        return int(core.sum())
+
     # -- the step -------------------------------------------------------------
     def step(self, return_stages=False):
         p, dt = self.p, self.dt

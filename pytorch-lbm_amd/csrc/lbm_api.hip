@@ -91,6 +91,12 @@ struct lbm_solver {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int gsteps = 0;
+  // droplet injection: pressure and chemical potential of the last step
+  // (internal order) kept for the observables until the next step, then the
+  // drop indicator; allocated at the first injection
+  void* keep = nullptr;
+  bool kept = false;
+  bool batched = false;       // owned by a batch engine
 };
 
 // ---------------------------------------------------------------------------
@@ -341,7 +347,7 @@ int lbm_destroy(lbm_solver* s) {
   void* ptrs[] = {s->Fb[0], s->Fb[1], s->Gb[0], s->Gb[1], s->u, s->phi[0], s->phi[1],
                   s->eps, s->ampf, s->meta, s->bnd, s->dist[0], s->dist[1], s->solid,
                   s->wmask, s->normal, s->surf, s->partials, s->mparts, s->dout, s->ctrl,
-                  s->chunks, s->chunk_flag};
+                  s->chunks, s->chunk_flag, s->keep};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete s;
@@ -535,6 +541,7 @@ int lbm_init_fields(lbm_solver* s, const void* phi_dev, const void* u_dev, void*
   s->local = true;
   s->prev_local = true;
   s->phiu_valid = false;
+  s->kept = false;
   return LBM_OK;
 }
 
@@ -619,6 +626,7 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     s->cur = dst;
     s->k += 1;
     s->phiu_valid = true;
+    s->kept = false;
   }
   HIPCHK(hipGetLastError());
   return LBM_OK;
@@ -880,7 +888,8 @@ __global__ void k_geom_obs(Grid g, int dim, const uint8_t* __restrict__ meta,
 }
 
 template <typename T, typename S, int DIM>
-static int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st) {
+static int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st,
+                          bool internal = false) {
   const Phys<T> ph = make_phys<T, DIM>(s);
   ObsArgs<T, S, DIM> a;
   std::memset(&a, 0, sizeof(a));
@@ -895,9 +904,22 @@ static int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st) {
   a.ctrl = s->ctrl;
   a.out = (T*)dst;
   a.slot = (int)(s->k & 1);
-  a.scaled = s->k > 0 ? 1 : 0;
+  // buffers in the local form (after init_fields or an injection) hold the
+  // observable state as it is; otherwise the last step's mass factor and
+  // boundary relax still apply
+  a.scaled = s->local ? 0 : 1;
   a.local = s->local ? 1 : 0;
+  a.internal = internal ? 1 : 0;
   const size_t N = (size_t)s->N;
+  if (s->kept && (which == LBM_FIELD_P || which == LBM_FIELD_MU)) {
+    // an injection leaves P and mu of the last step as they were
+    const char* kp = (const char*)s->keep + (which == LBM_FIELD_MU ? N * s->esz : 0);
+    if (internal) {
+      HIPCHK(hipMemcpyAsync(dst, kp, N * s->esz, hipMemcpyDeviceToDevice, st));
+      return LBM_OK;
+    }
+    return to_public(s, kp, dst, s->esz, 1, st);
+  }
   switch (which) {
     case LBM_FIELD_PHI: a.kind = OBS_PHI; break;
     case LBM_FIELD_PHI_RAW:
@@ -979,7 +1001,7 @@ int lbm_get_mass(lbm_solver* s, double* m0, double* m_last, void* stream) {
   HIPCHK(hipMemcpyAsync(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (m0) *m0 = c.M0;
-  if (m_last) *m_last = s->k > 0 ? c.mass[s->k & 1] : c.M0;
+  if (m_last) *m_last = s->local ? c.M0 : c.mass[s->k & 1];
   return LBM_OK;
 }
 
@@ -992,12 +1014,81 @@ int lbm_set_mass_target(lbm_solver* s, double m0) {
 long long lbm_step_count(const lbm_solver* s) { return s ? s->k : -1; }
 
 // ---------------------------------------------------------------------------
+// Mid-run droplet injection (lbm/fe_ac_solver.py:645-732).
+// ---------------------------------------------------------------------------
+template <typename T, typename S, int DIM>
+static int inject_impl(lbm_solver* s, const void* c_pub, const double* uimp, hipStream_t st) {
+  const Phys<T> ph = make_phys<T, DIM>(s);
+  const size_t fb = (size_t)s->N * s->esz;
+  if (!s->keep) HIPCHK(hipMalloc(&s->keep, 3 * fb));
+  char* kp = (char*)s->keep;
+  if (!s->kept) {
+    // the injection changes phi; P and mu stay those of the last step
+    int rc = get_field_impl<T, S, DIM>(s, LBM_FIELD_P, kp, st, true);
+    if (!rc) rc = get_field_impl<T, S, DIM>(s, LBM_FIELD_MU, kp + fb, st, true);
+    if (rc) return rc;
+  }
+  int rc = to_internal(s, c_pub, kp + 2 * fb, s->esz, 1, st);
+  if (rc) return rc;
+  const int pslot = (int)(s->k & 1), oslot = pslot ^ 1;
+  InjectArgs<T, S, DIM> a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = s->g;
+  a.Fin = (const S*)s->Fb[s->cur];
+  a.Gin = (const S*)s->Gb[s->cur];
+  a.Fout = (S*)s->Fb[s->cur ^ 1];
+  a.Gout = (S*)s->Gb[s->cur ^ 1];
+  a.pref = (const T*)s->phi[pslot];
+  a.C = (const T*)(kp + 2 * fb);
+  a.P = (const T*)kp;
+  a.u = (T*)s->u;
+  // neighbours read phi[pslot] as their 16-bit reference: the new phi goes
+  // to the other slot first and is copied over after the pass
+  a.phi_out = (T*)s->phi[oslot];
+  a.meta = s->meta;
+  a.eps = (const T*)s->eps;
+  a.ctrl = s->ctrl;
+  a.partials = s->partials;
+  for (int d = 0; d < DIM; ++d) a.uimp[d] = uimp ? (T)uimp[d] : T(0);
+  a.slot = pslot;
+  if (s->local)
+    k_inject<T, S, DIM, BS, true><<<s->nblocks, BS, 0, st>>>(a, ph);
+  else
+    k_inject<T, S, DIM, BS, false><<<s->nblocks, BS, 0, st>>>(a, ph);
+  k_sum_partials<BS><<<1, BS, 0, st>>>(s->partials, (unsigned int)((s->N + 63) / 64),
+                                       &s->ctrl->M0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(s->phi[pslot], s->phi[oslot], fb, hipMemcpyDeviceToDevice, st));
+  return LBM_OK;
+}
+
+int lbm_inject_droplet(lbm_solver* s, const void* c_dev, const double* u_impact, void* stream) {
+  if (!s || !c_dev) return fail(LBM_EINVAL, "null argument");
+  if (!s->inited) return fail(LBM_ESTATE, "inject_droplet() before init_fields()");
+  if (s->batched) return fail(LBM_EINVAL, "cases of a batch engine take no injections");
+#define CALL(T, S, D) inject_impl<T, S, D>(s, c_dev, u_impact, (hipStream_t)stream)
+  int rc = DISPATCH3(s, CALL);
+#undef CALL
+  if (rc) return rc;
+  s->cur ^= 1;
+  s->local = true;
+  s->phiu_valid = false;
+  s->kept = true;
+  // a captured graph assumes the post-collision form of the buffers
+  if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+  if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+  s->gsteps = 0;
+  return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Solver state snapshot (preemption / checkpoint).  Layout of the byte image:
 //   [0, 256)   StateHeader
 //   then       F[cur], G[cur], G[cur^1] (AoSoA, the step's distributions and
 //              the previous g set the P observable is pulled from), u,
-//              phi[0], phi[1], Ctrl
-// each array starting at a 256-byte boundary.  Geometry is not included: the
+//              phi[0], phi[1], Ctrl, kept P, kept mu
+// each array starting at a 256-byte boundary.  The kept P and mu (observables
+// right after a droplet injection) are copied only when the header says so.  Geometry is not included: the
 // restoring solver gets the same set_geometry first.
 // ---------------------------------------------------------------------------
 namespace {
@@ -1005,16 +1096,16 @@ struct StateHeader {
   uint32_t magic, version;
   int32_t nx, ny, nz, precision;
   long long k;
-  int32_t cur, local, prev_local, phiu_valid;
+  int32_t cur, local, prev_local, phiu_valid, kept, pad;
   uint64_t bytes;
 };
 constexpr uint32_t kStateMagic = 0x4c424d53u;   // "LBMS"
-constexpr uint32_t kStateVersion = 1;
+constexpr uint32_t kStateVersion = 2;
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 struct StateLayout {
-  size_t dist, u, phi, off[8], total;
+  size_t dist, u, phi, off[9], total;
 };
 
 StateLayout state_layout(const lbm_solver* s) {
@@ -1022,9 +1113,9 @@ StateLayout state_layout(const lbm_solver* s) {
   L.dist = (size_t)s->Q * (size_t)padded_nodes(s->N) * s->dsz;
   L.u = (size_t)s->dim * (size_t)s->N * s->esz;
   L.phi = (size_t)s->N * s->esz;
-  const size_t sz[7] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl)};
+  const size_t sz[9] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl), L.phi, L.phi};
   size_t o = 256;
-  for (int i = 0; i < 7; ++i) {
+  for (int i = 0; i < 9; ++i) {
     L.off[i] = o;
     o += align256(sz[i]);
   }
@@ -1046,12 +1137,13 @@ int lbm_save_state(lbm_solver* s, void* dst, void* stream) {
   h.nx = s->p.nx; h.ny = s->p.ny; h.nz = s->p.nz; h.precision = s->p.precision;
   h.k = s->k;
   h.cur = s->cur; h.local = s->local; h.prev_local = s->prev_local; h.phiu_valid = s->phiu_valid;
+  h.kept = s->kept;
   h.bytes = L.total;
   char* d = (char*)dst;
-  const void* src[7] = {s->Fb[s->cur], s->Gb[s->cur], s->Gb[s->cur ^ 1], s->u, s->phi[0],
-                        s->phi[1], s->ctrl};
-  const size_t sz[7] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl)};
-  for (int i = 0; i < 7; ++i)
+  const void* src[9] = {s->Fb[s->cur], s->Gb[s->cur], s->Gb[s->cur ^ 1], s->u, s->phi[0],
+                        s->phi[1], s->ctrl, s->keep, (const char*)s->keep + L.phi};
+  const size_t sz[9] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl), L.phi, L.phi};
+  for (int i = 0; i < (s->kept ? 9 : 7); ++i)
     HIPCHK(hipMemcpyAsync(d + L.off[i], src[i], sz[i], hipMemcpyDefault, st));
   // the header comes from host memory that goes out of scope: synchronous copy
   HIPCHK(hipStreamSynchronize(st));
@@ -1073,17 +1165,19 @@ int lbm_load_state(lbm_solver* s, const void* src, void* stream) {
       h.bytes != L.total)
     return fail(LBM_EINVAL, "state image is for another grid or precision");
   if (h.cur < 0 || h.cur > 1) return fail(LBM_EINVAL, "corrupt state image");
+  if (h.kept && !s->keep) HIPCHK(hipMalloc(&s->keep, 3 * L.phi));
   const char* d = (const char*)src;
-  void* dstp[7] = {s->Fb[h.cur], s->Gb[h.cur], s->Gb[h.cur ^ 1], s->u, s->phi[0], s->phi[1],
-                   s->ctrl};
-  const size_t sz[7] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl)};
-  for (int i = 0; i < 7; ++i)
+  void* dstp[9] = {s->Fb[h.cur], s->Gb[h.cur], s->Gb[h.cur ^ 1], s->u, s->phi[0], s->phi[1],
+                   s->ctrl, s->keep, (char*)s->keep + L.phi};
+  const size_t sz[9] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl), L.phi, L.phi};
+  for (int i = 0; i < (h.kept ? 9 : 7); ++i)
     HIPCHK(hipMemcpyAsync(dstp[i], d + L.off[i], sz[i], hipMemcpyDefault, st));
   s->k = h.k;
   s->cur = h.cur;
   s->local = h.local != 0;
   s->prev_local = h.prev_local != 0;
   s->phiu_valid = h.phiu_valid != 0;
+  s->kept = h.kept != 0;
   s->inited = true;
   if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
   if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
@@ -1097,7 +1191,7 @@ static int measure_impl(lbm_solver* s, double* out, hipStream_t st) {
   const unsigned int nb = (unsigned int)((s->N + (long long)BS * kMeasPer - 1) / ((long long)BS * kMeasPer));
   k_measure<T, DIM, BS><<<nb, BS, 0, st>>>(
       s->g, ph, (const T*)s->phi[s->k & 1], s->meta, s->ctrl, (int)(s->k & 1),
-      s->k > 0 ? 1 : 0, (MeasRec*)s->mparts);
+      s->local ? 0 : 1, (MeasRec*)s->mparts);
   k_measure_final<DIM, BS><<<1, BS, 0, st>>>((const MeasRec*)s->mparts, nb, s->dout);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->dout, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1119,7 +1213,7 @@ static int contact_impl(lbm_solver* s, double cx, double cy, double* out, hipStr
   const long long ncol = (long long)s->g.nx * s->g.ny;
   k_contact<T, DIM, BS><<<nblk(ncol), BS, 0, st>>>(
       s->g, ph, (const T*)s->phi[s->k & 1], s->meta, s->surf, s->ctrl, (int)(s->k & 1),
-      s->k > 0 ? 1 : 0, cx, cy, (double*)s->mparts);
+      s->local ? 0 : 1, cx, cy, (double*)s->mparts);
   k_contact_final<BS><<<1, BS, 0, st>>>((const double*)s->mparts, nblk(ncol), s->dout + 12);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->dout + 12, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1294,6 +1388,7 @@ int lbm_batch_add(lbm_batch* b, const lbm_case_spec* sp, void* stream, int* case
   hipStream_t st = (hipStream_t)stream;
   lbm_solver* s = nullptr;
   int rc = create_impl(&sp->p, &s, true, st);
+  if (!rc) s->batched = true;
   if (rc) return rc;
   rc = s->dim == 3 ? gen_case<3>(b, s, *sp, st) : gen_case<2>(b, s, *sp, st);
   if (rc) { (void)hipStreamSynchronize(st); lbm_destroy(s); return rc; }
@@ -1487,6 +1582,7 @@ static void batch_host_advance(lbm_batch* b) {
     s->cur ^= 1;
     s->k += 1;
     s->phiu_valid = true;
+    s->kept = false;
   }
 }
 

@@ -985,6 +985,7 @@ template <typename T, typename S, int DIM> struct ObsArgs {
   int slot;        // mass factor slot of the step that produced phi_cur
   int scaled;      // apply mass factor / relax (phi, f); 0 right after init
   int init_lap;    // mu with the init_fields Laplacian
+  int internal;    // write in the internal node order instead of the public one
 };
 
 template <typename T, typename S, int DIM, int BS>
@@ -996,7 +997,8 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
   if (x >= N) return;
   const uint8_t m = a.meta[x];
   const Coord<DIM> c = decompose<DIM>(g, x);
-  const uint32_t ex = ext_index<DIM>(g, c);   // outputs in the public order
+  // outputs in the public order unless asked otherwise
+  const uint32_t ex = a.internal ? x : ext_index<DIM>(g, c);
   switch (a.kind) {
     case OBS_PHI: {
       T p = a.phi_cur[x];
@@ -1082,6 +1084,104 @@ __global__ __launch_bounds__(BS) void k_gradlap(Grid g, Phys<T> ph, const T* __r
 #pragma unroll
   for (int d = 0; d < DIM; ++d) grad[(long long)d * N + ex] = gr[d];
   lap[ex] = l;
+}
+
+// ---------------------------------------------------------------------------
+// Mid-run droplet injection (lbm/fe_ac_solver.py:645-732), one pass over the
+// nodes.  The current state is observed exactly as the phi / u / f / g
+// observables report it (pull, mass factor and boundary relax of the last
+// step), edited, and stored back in the post-stream ("local") form at the
+// other buffer set, so the next step reads it in place as it does after
+// init_fields:
+//   core = C > 0.1;  phi' = core ? max(phi, C) : phi, clamped, 0 in solid;
+//   u' = (1 - C) u + C u_impact, 0 in solid;
+//   core nodes: f = f_eq(phi', u'), g = g_eq(P, u') with the P of the last
+//   step; other nodes keep their f, g.
+// phi' goes to phi_out; per-wave fp64 partial sums of phi' give the new mass
+// reference.  16-bit f slots are stored relative to w_i phi'(x), the
+// reference the next step's pull adds back.
+// ---------------------------------------------------------------------------
+template <typename T, typename S, int DIM> struct InjectArgs {
+  Grid g;
+  const S* __restrict__ Fin;    // buffer set holding the current state
+  const S* __restrict__ Gin;
+  S* __restrict__ Fout;         // the other set, written in the local form
+  S* __restrict__ Gout;
+  const T* __restrict__ pref;   // start-of-step phi of the last step (16-bit f reference)
+  const T* __restrict__ C;      // new drop indicator, internal order
+  const T* __restrict__ P;      // pressure of the last step, internal order
+  T* __restrict__ u;            // velocity, updated in place
+  T* __restrict__ phi_out;
+  const uint8_t* __restrict__ meta;
+  const T* __restrict__ eps;
+  const Ctrl* ctrl;
+  double* __restrict__ partials;
+  T uimp[3];
+  int slot;                     // mass factor slot of the last step
+};
+
+template <typename T, typename S, int DIM, int BS, bool LOCAL>
+__global__ __launch_bounds__(BS) void k_inject(InjectArgs<T, S, DIM> a, Phys<T> ph) {
+  constexpr int Q = Lat<DIM>::Q;
+  constexpr bool DEV = !std::is_same<T, S>::value;
+  const Grid& g = a.g;
+  const long long N = g.N;
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  double v = 0.0;
+  if (x < N) {
+    const uint8_t m = a.meta[x];
+    const Coord<DIM> c = decompose<DIM>(g, x);
+    const Nbrs<DIM> nb = neighbours<DIM>(g, x, c);
+    const Buf bF = dist_buf<DIM>(a.Fin, N), bG = dist_buf<DIM>(a.Gin, N);
+    const Buf bR = field_buf(a.pref, N);
+    T f[Q], gg[Q];
+    pull<T, S, DIM, DEV, true, LOCAL>(bF, bG, x, nb, m, a.eps, bR, ph, f, gg);
+    T p = a.pref[x];
+    if constexpr (!LOCAL) {
+      const T cf = (T)a.ctrl->c[a.slot];
+      const T praw = p;
+      p = clamp01(praw * cf);
+      scale_f<T, DIM>(ph, f, cf, m, (m & M_TOP) ? praw : T(0), &p);
+    }
+    const T C = a.C[x];
+    const bool core = C > T(0.1);
+    const bool solid = (m & M_SOLID) != 0;
+    T pn = (core && p < C) ? C : p;
+    pn = clamp01(pn);
+    if (solid) pn = T(0);
+    T uu[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      T* up = a.u + (long long)d * N + x;
+      T un = (T(1) - C) * *up + C * a.uimp[d];
+      if (solid) un = T(0);
+      uu[d] = un;
+      *up = un;
+    }
+    if (core) {
+      const T usq_t = divk(vsq<DIM>(uu), ph.two_cs2, ph.rcp_two_cs2);
+      T ec[Q];
+      ediv<T, DIM>(ph, uu, ph.cs2, ph.rcp_cs2, ec);
+      const T P = a.P[x];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        f[i] = feq_of<T, DIM>(ph, i, pn, ec[i], usq_t);
+        gg[i] = geq_of<T, DIM>(ph, i, P, ec[i], usq_t);
+      }
+    }
+    const Buf oF = dist_buf<DIM>(a.Fout, N), oG = dist_buf<DIM>(a.Gout, N);
+    const uint32_t vx = dvoff<Q, S>(x);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      T fv = f[i];
+      if constexpr (DEV) fv = fv - ph.template w<DIM>(i) * pn;
+      dput<Q, T, S>(oF, i, vx, fv);
+      dput<Q, T, S>(oG, i, vx, gg[i]);
+    }
+    a.phi_out[x] = pn;
+    v = (double)pn;
+  }
+  wave_partial(v, x, N, a.partials);
 }
 
 // Reordering between the public layout (last axis fastest) and the internal

@@ -1,9 +1,9 @@
 """AllenCahnSolver: the reference's public class (lbm/fe_ac_solver.py:51-732),
 backed by the HIP kernels of liblbm_amd.so.
 
-Constructor, ``set_solid``, ``init_fields``, ``step`` and the private
-``_lattice_gradient_and_laplacian`` keep the reference's signatures and
-semantics.  All arithmetic runs on the device (pytorch-lbm_amd/csrc); this
+Constructor, ``set_solid``, ``init_fields``, ``step``, ``inject_droplet`` and
+the private ``_lattice_gradient_and_laplacian`` keep the reference's
+signatures and semantics.  All arithmetic runs on the device (pytorch-lbm_amd/csrc); this
 class only moves inputs to the device, calls the C ABI on torch's current
 stream and materialises the observable attributes (f, g, phi, rho, u, P, p,
 mu_phi) on first access after each step.  There is no CPU path: the solver
@@ -412,4 +412,41 @@ class AllenCahnSolver:
         return out.value
 
     def inject_droplet(self, center, radius, xi, u_impact=None):
-        raise NotImplementedError("inject_droplet is not built on the HIP path")
+        """Add a droplet between steps (lbm/fe_ac_solver.py:645-732).
+
+        C = clip(0.5 + 0.5 tanh(2 (radius - |x - center|) / xi), 0, 1) is built
+        on the host in fp64 and rounded to float32, as the reference does;
+        the state edit (phi = max(phi, C) and equilibrium f, g where C > 0.1,
+        u blended towards u_impact with weight C, new mass target) is one
+        device pass.  P and mu_phi keep their last-step values until the next
+        step."""
+        ndim = self.ndim
+        shape = tuple(self.shape)
+        if u_impact is None:
+            u_impact = (0.0,) * ndim
+        if len(center) != ndim or len(u_impact) != ndim:
+            raise ValueError(f"center and u_impact need {ndim} components")
+        mesh = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+        dist = np.sqrt(sum((M - center[d]) ** 2 for d, M in enumerate(mesh)))
+        C = np.clip(0.5 + 0.5 * np.tanh(2.0 * (radius - dist) / xi), 0.0, 1.0)
+        C = C.astype(np.float32)
+        dt = self._compute_dtype
+        c_t = torch.from_numpy(C).to(device=self.device, dtype=dt)
+        np_dt = np.float64 if dt == torch.float64 else np.float32
+        n_core = int(np.count_nonzero(C.astype(np_dt) > np_dt(0.1)))
+        uimp = (ctypes.c_double * 3)(*[float(v) for v in u_impact] + [0.0] * (3 - ndim))
+        with torch.cuda.device(self.device):
+            nat.check(nat.lib.lbm_inject_droplet(self._h, nat.ptr(c_t), uimp, self._stream()),
+                      "lbm_inject_droplet")
+        self._cache.clear()
+        self._overrides.clear()
+        m0 = ctypes.c_double()
+        nat.check(nat.lib.lbm_get_mass(self._h, ctypes.byref(m0), None, self._stream()),
+                  "lbm_get_mass")
+        self.phi_mass_init = m0.value
+        self._phi_u_prev = None
+        center_str = ",".join(f"{c:.0f}" for c in center)
+        vel_str = ",".join(f"{u:.3f}" for u in u_impact)
+        print(f"  [inject] Droplet at ({center_str}) R={radius:.1f} u=({vel_str}) "
+              f"core_nodes={n_core}")
+        return n_core
