@@ -32,6 +32,9 @@
 
 namespace lbm {
 
+// Cache-policy operand of buffer loads and stores: the NT (non-temporal) bit.
+constexpr int kNT = 2;
+
 struct Ctrl {
   double M0;
   double mass[2];     // fp64 sum of the start-of-step phi, by step parity
@@ -87,41 +90,41 @@ struct Buf {
 
 template <typename E> struct BufIO;
 template <> struct BufIO<float> {
-  __device__ static __forceinline__ float ld(const Buf& b, uint32_t vo, uint32_t so) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)vo, (int)so, 0));
+  template <int CP = 0> __device__ static __forceinline__ float ld(const Buf& b, uint32_t vo, uint32_t so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)vo, (int)so, CP));
   }
-  __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so, float v) {
+  template <int CP = 0> __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), b.r, (int)vo,
-                                          (int)so, 0);
+                                          (int)so, CP);
   }
 };
 template <> struct BufIO<double> {
-  __device__ static __forceinline__ double ld(const Buf& b, uint32_t vo, uint32_t so) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)vo, (int)so, 0));
+  template <int CP = 0> __device__ static __forceinline__ double ld(const Buf& b, uint32_t vo, uint32_t so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)vo, (int)so, CP));
   }
-  __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so, double v) {
+  template <int CP = 0> __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so, double v) {
     typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), b.r, (int)vo, (int)so, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), b.r, (int)vo, (int)so, CP);
   }
 };
 template <> struct BufIO<__hip_bfloat16> {
-  __device__ static __forceinline__ __hip_bfloat16 ld(const Buf& b, uint32_t vo, uint32_t so) {
+  template <int CP = 0> __device__ static __forceinline__ __hip_bfloat16 ld(const Buf& b, uint32_t vo, uint32_t so) {
     return __builtin_bit_cast(__hip_bfloat16,
-                              __builtin_amdgcn_raw_buffer_load_b16(b.r, (int)vo, (int)so, 0));
+                              __builtin_amdgcn_raw_buffer_load_b16(b.r, (int)vo, (int)so, CP));
   }
-  __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so,
+  template <int CP = 0> __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so,
                                             __hip_bfloat16 v) {
     __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), b.r, (int)vo,
-                                          (int)so, 0);
+                                          (int)so, CP);
   }
 };
 template <> struct BufIO<__half> {
-  __device__ static __forceinline__ __half ld(const Buf& b, uint32_t vo, uint32_t so) {
-    return __builtin_bit_cast(__half, __builtin_amdgcn_raw_buffer_load_b16(b.r, (int)vo, (int)so, 0));
+  template <int CP = 0> __device__ static __forceinline__ __half ld(const Buf& b, uint32_t vo, uint32_t so) {
+    return __builtin_bit_cast(__half, __builtin_amdgcn_raw_buffer_load_b16(b.r, (int)vo, (int)so, CP));
   }
-  __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so, __half v) {
+  template <int CP = 0> __device__ static __forceinline__ void st(const Buf& b, uint32_t vo, uint32_t so, __half v) {
     __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), b.r, (int)vo,
-                                          (int)so, 0);
+                                          (int)so, CP);
   }
 };
 
@@ -131,10 +134,10 @@ __device__ __forceinline__ T bget(const Buf& b, uint32_t N, int i, uint32_t idx)
   return Stor<S>::template to<T>(
       BufIO<S>::ld(b, idx * (uint32_t)sizeof(S), (uint32_t)i * N * (uint32_t)sizeof(S)));
 }
-template <typename T, typename S>
+template <typename T, typename S, int CP = 0>
 __device__ __forceinline__ void bput(const Buf& b, uint32_t N, int i, uint32_t idx, T v) {
-  BufIO<S>::st(b, idx * (uint32_t)sizeof(S), (uint32_t)i * N * (uint32_t)sizeof(S),
-               Stor<S>::template from<T>(v));
+  BufIO<S>::template st<CP>(b, idx * (uint32_t)sizeof(S), (uint32_t)i * N * (uint32_t)sizeof(S),
+                            Stor<S>::template from<T>(v));
 }
 
 // Distribution layout (AoSoA): nodes are grouped in blocks of 64 (one wave of
@@ -152,9 +155,10 @@ template <int Q, typename S> __device__ __forceinline__ uint32_t dvoff(uint32_t 
   return __umul24(idx >> 6, (uint32_t)((Q - 1) * 64 * sizeof(S))) + idx * (uint32_t)sizeof(S);
 }
 // load of direction i at the node whose dvoff is vo
-template <int Q, typename T, typename S>
+template <int Q, typename T, typename S, int CP = 0>
 __device__ __forceinline__ T dload(const Buf& b, int i, uint32_t vo) {
-  return Stor<S>::template to<T>(BufIO<S>::ld(b, vo, (uint32_t)i * 64u * (uint32_t)sizeof(S)));
+  return Stor<S>::template to<T>(
+      BufIO<S>::template ld<CP>(b, vo, (uint32_t)i * 64u * (uint32_t)sizeof(S)));
 }
 template <int Q, typename T, typename S>
 __device__ __forceinline__ T dget(const Buf& b, int i, uint32_t idx) {
@@ -162,9 +166,10 @@ __device__ __forceinline__ T dget(const Buf& b, int i, uint32_t idx) {
       BufIO<S>::ld(b, dvoff<Q, S>(idx), (uint32_t)i * 64u * (uint32_t)sizeof(S)));
 }
 // store of direction i at the node whose dvoff is vo
-template <int Q, typename T, typename S>
+template <int Q, typename T, typename S, int CP = 0>
 __device__ __forceinline__ void dput(const Buf& b, int i, uint32_t vo, T v) {
-  BufIO<S>::st(b, vo, (uint32_t)i * 64u * (uint32_t)sizeof(S), Stor<S>::template from<T>(v));
+  BufIO<S>::template st<CP>(b, vo, (uint32_t)i * 64u * (uint32_t)sizeof(S),
+                            Stor<S>::template from<T>(v));
 }
 
 // Correctly rounded division by a divisor d whose reciprocal r = RN(1/d) is
@@ -283,7 +288,7 @@ __device__ __forceinline__ Nbrs<DIM> neighbours(const Grid& g, uint32_t x, const
 // LOCAL: the buffers hold post-stream values (first step after init_fields)
 // and are read in place.  It is a template parameter so that the common form
 // has no branch between the geometry-byte load and the pulls.
-template <typename T, typename S, int DIM, bool DEV, bool WITH_G, bool LOCAL>
+template <typename T, typename S, int DIM, bool DEV, bool WITH_G, bool LOCAL, int CP = 0>
 __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, const Nbrs<DIM>& nb,
                                      uint8_t m, const T* __restrict__ eps_arr,
                                      const Buf& pref, const Phys<T>& ph, T (&f)[Lat<DIM>::Q],
@@ -310,8 +315,8 @@ __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, con
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const uint32_t vs = dvoff<Q, S>(nb.i[OPP<DIM>(i)]);
-    f[i] = dload<Q, T, S>(F, i, vs);
-    if constexpr (WITH_G) gg[i] = dload<Q, T, S>(G, i, vs);
+    f[i] = dload<Q, T, S, CP>(F, i, vs);
+    if constexpr (WITH_G) gg[i] = dload<Q, T, S, CP>(G, i, vs);
   }
   // nothing that reads m may be scheduled above the loads
   __builtin_amdgcn_sched_barrier(0);
@@ -717,12 +722,19 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   // g is pulled first and f only for the final f collision, after g* and u
   // are stored: f's 19 values are then never live together with g's, which
   // keeps the register footprint (and so the occupancy) of this pass down.
+  // Memory-side cache (256 MB on MI355X): pass A reads back the f set this
+  // pass writes, so f stores keep the default policy; the g set written here
+  // is not read before the next pass B, and the source sets die with their
+  // last read here, so those accesses are non-temporal and leave the cache to
+  // the new f.  (For 16-bit storage, whose whole state nearly fits, the NT
+  // f reads measured slower and stay default.)
+  constexpr int CP_G = kNT, CP_FLD = sizeof(S) >= 4 ? kNT : 0;
   T gg[Q], unused[Q];
   const uint32_t N32 = (uint32_t)N;
   const Buf bFi = dist_buf<DIM>(a.Fin, N), bGi = dist_buf<DIM>(a.Gin, N);
   const Buf bPp = field_buf(a.phi_prev, N), bPc = field_buf(a.phi_cur, N);
   const Buf bU = field_buf(a.u, N, DIM);
-  pull<T, S, DIM, false, false, LOCAL>(bGi, bGi, x, nb, m, a.eps, bPp, ph, gg, unused);
+  pull<T, S, DIM, false, false, LOCAL, CP_G>(bGi, bGi, x, nb, m, a.eps, bPp, ph, gg, unused);
   T pprev = T(0);
   if (((m & M_TOP) && !LOCAL) || a.phiu_valid) pprev = a.phi_prev[x];
 
@@ -880,7 +892,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
     for (int i = 0; i < Q; ++i) {
       const T ge = geq_of<T, DIM>(ph, i, P, ec[i], usq_n);
       const T gs = (gg[i] - omg * (gg[i] - ge)) + G[i];
-      dput<Q, T, S>(bGo, i, vx, gs);
+      dput<Q, T, S, CP_G>(bGo, i, vx, gs);
     }
   }
 #pragma unroll
@@ -895,7 +907,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   asm volatile("" : "+v"(x2));
   const Nbrs<DIM> nb2 = neighbours<DIM>(g, x2, decompose<DIM>(g, x2));
   T f[Q];
-  pull<T, S, DIM, DEV, false, LOCAL>(bFi, bFi, x, nb2, m, a.eps, bPp, ph, f, unused);
+  pull<T, S, DIM, DEV, false, LOCAL, CP_FLD>(bFi, bFi, x, nb2, m, a.eps, bPp, ph, f, unused);
   if constexpr (!LOCAL) scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot_prev], m, pprev, nullptr);
   T fsrc[Q];
   fsrc[0] = T(0);
