@@ -23,7 +23,18 @@ def config_kwargs(m):
               xi=4.0, tau_l=0.53, tau_g=0.53, theta_eq=m["theta"])
     if m["tau_h"] is not None:
         kw["tau_h"] = m["tau_h"]
+    if "g_force" in m:
+        kw["g_force"] = tuple(m["g_force"]) + (0.0,) * (3 - len(m["g_force"]))
     return kw
+
+
+def inject_event(m):
+    """(step after which the case injects a droplet, inject_droplet args) or None."""
+    inj = m.get("inject")
+    if not inj:
+        return None
+    return inj["after"], (tuple(inj["center"]), float(inj["radius"]), float(inj["xi"]),
+                          tuple(inj["u"]))
 
 
 def solver_kwargs(m):

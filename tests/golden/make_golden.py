@@ -17,7 +17,10 @@ routines and invariants in tests/test_oracle_*.py):
   * the wetting precompute outputs (lbm/fe_wetting.py:122-367) for each
     substrate type, including the discrete-normal path;
   * ``_lattice_gradient_and_laplacian`` + ``correct_gradient`` on the
-    step-1 phi (lbm/fe_ac_solver.py:296-320, lbm/fe_wetting.py:448-539).
+    step-1 phi (lbm/fe_ac_solver.py:296-320, lbm/fe_wetting.py:448-539);
+  * the stab_mode variants and the body force of the force stage
+    (lbm/fe_ac_solver.py:415-439);
+  * ``inject_droplet`` between two steps (lbm/fe_ac_solver.py:645-732).
 
 Usage (in the build container only):  python tests/golden/make_golden.py
 """
@@ -107,6 +110,33 @@ CASES = {
         grid=(32, 32, 1), sub="flat", R_star=None, R_d=None, theta=90.0,
         kw=dict(), dtype="float32", center=(16.0, 9.0), radius=6.0,
         u=(0.0, 0.0), frac=True, tau_h=None, fg_steps=()),
+    # force-stage variants: Liang (no viscous correction), none (surface
+    # tension only) with a body force
+    "ridge120_liang_2d_f64": dict(
+        grid=(32, 32, 1), sub="ridge", R_star=1.0, R_d=5.0, theta=120.0,
+        kw=dict(geo_amplification=1.5, geometric_wetting=True, stab_mode="liang"),
+        dtype="float64", center=(16.0, 13.0), radius=5.0,
+        u=(0.0, -0.04), frac=True, tau_h=None, fg_steps=(1,)),
+    "flat144_none_grav_2d_f64": dict(
+        grid=(32, 32, 1), sub="flat", R_star=None, R_d=None, theta=144.0,
+        kw=dict(geo_amplification=1.5, geometric_wetting=True, stab_mode="none"),
+        dtype="float64", center=(16.0, 12.0), radius=7.0,
+        u=(0.0, -0.02), frac=True, tau_h=None, fg_steps=(1,), g_force=(0.0, -1e-5)),
+    # a second drop injected after step 3 (3D, fp64) and after step 2 (2D, fp32)
+    "ridge162_inject_f64": dict(
+        grid=(20, 20, 16), sub="ridge", R_star=1.0, R_d=5.0, theta=162.0,
+        kw=dict(geo_amplification=1.5, geometric_wetting=True),
+        dtype="float64", center=(10.0, 10.0, 10.0), radius=4.0,
+        u=(0.0, 0.0, -0.05), frac=True, tau_h=None, fg_steps=(),
+        inject=dict(after=3, center=(5.0, 13.0, 12.0), radius=3.0, xi=4.0,
+                    u=(0.01, 0.0, -0.04))),
+    "flat162_inject_2d_f32": dict(
+        grid=(32, 32, 1), sub="flat", R_star=None, R_d=None, theta=162.0,
+        kw=dict(geo_amplification=1.5, geometric_wetting=True),
+        dtype="float32", center=(12.0, 9.0), radius=5.0,
+        u=(0.0, -0.03), frac=True, tau_h=None, fg_steps=(2,),
+        inject=dict(after=2, center=(22.0, 10.0), radius=4.0, xi=4.0,
+                    u=(0.0, -0.05))),
 }
 
 SAVE_STEPS = (1, 2, 5, 20, 60)
@@ -120,6 +150,8 @@ def run_case(name, c):
                   theta_eq=c["theta"], device="cpu")
     if c["tau_h"] is not None:
         cfg_kw["tau_h"] = c["tau_h"]
+    if "g_force" in c:
+        cfg_kw["g_force"] = tuple(c["g_force"]) + (0.0,) * (3 - len(c["g_force"]))
     cfg = FEConfig(**cfg_kw)
     out = {}
     with contextlib.redirect_stdout(io.StringIO()):
@@ -167,9 +199,17 @@ def run_case(name, c):
     if w is not None and sol.geometric_wetting:
         out["op_grad_corr"] = w.correct_gradient(grad.clone()).numpy()
     # trajectory
+    inj = c.get("inject")
     with contextlib.redirect_stdout(io.StringIO()):
         for s in range(1, max(SAVE_STEPS) + 1):
             sol.step()
+            if inj is not None and s == inj["after"]:
+                sol.inject_droplet(inj["center"], inj["radius"], inj["xi"], inj["u"])
+                out["inj_phi"] = sol.phi.numpy().copy()
+                out["inj_u"] = sol.u.numpy().copy()
+                out["inj_f"] = sol.f.numpy().copy()
+                out["inj_g"] = sol.g.numpy().copy()
+                out["inj_M0"] = np.array(sol.phi_mass_init, dtype=np.float64)
             if s in SAVE_STEPS:
                 out[f"s{s}_phi"] = sol.phi.numpy().copy()
                 out[f"s{s}_u"] = sol.u.numpy().copy()

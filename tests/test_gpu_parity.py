@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 import torch
 
-from _cases import case_names, config_kwargs, load, rel, solver_kwargs
+from _cases import case_names, config_kwargs, inject_event, load, rel, solver_kwargs
 from oracle.ac_oracle import OracleSolver, Params
 
 pytestmark = pytest.mark.gpu
@@ -109,11 +109,18 @@ def test_trajectory(name):
     o = oracle(name, np.float32 if f32 else np.float64)
     t64 = oracle(name, np.float64) if f32 else None
     fluid = ~z["solid"]
+    ev = inject_event(m)
     for step in range(1, max(m["save_steps"]) + 1):
         s.step()
         o.step()
         if t64 is not None:
             t64.step()
+        if ev is not None and step == ev[0]:
+            s.inject_droplet(*ev[1])
+            o.inject_droplet(*ev[1])
+            if t64 is not None:
+                t64.inject_droplet(*ev[1])
+            assert rel(np_(s.phi), z["inj_phi"]) < (1e-6 if f32 else 1e-12)
         if step not in m["save_steps"]:
             continue
         for key, a, ob, fl, mk in (("phi", "phi", "phi", 0.0, fluid),

@@ -11,6 +11,7 @@ import os
 import numpy as np
 import pytest
 
+from _cases import inject_event
 from oracle.ac_oracle import OracleSolver, Params
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -31,6 +32,8 @@ def build(name):
               xi=4.0, tau_l=0.53, tau_g=0.53, theta_eq=m["theta"])
     if m["tau_h"] is not None:
         kw["tau_h"] = m["tau_h"]
+    if "g_force" in m:
+        kw["g_force"] = tuple(m["g_force"]) + (0.0,) * (3 - len(m["g_force"]))
     p = Params(**kw)
     dt = np.float32 if m["dtype"] == "float32" else np.float64
     skw = dict(m["kw"])
@@ -135,10 +138,21 @@ def test_trajectory(name):
     f32 = m["dtype"] == "float32"
     t = _twin64(name) if f32 else None
     fluid = ~o.solid
+    ev = inject_event(m)
     for s in range(1, max(m["save_steps"]) + 1):
         o.step()
         if t is not None:
             t.step()
+        if ev is not None and s == ev[0]:
+            o.inject_droplet(*ev[1])
+            if t is not None:
+                t.inject_droplet(*ev[1])
+            tol = 1e-6 if f32 else 1e-12
+            assert err(o.phi, z["inj_phi"]) < tol
+            assert err(o.u, z["inj_u"]) < tol
+            assert err(o.f, z["inj_f"]) < tol
+            assert err(o.g, z["inj_g"], floor=1e-5) < tol
+            assert abs(o.M0 - float(z["inj_M0"])) < (1e-6 if f32 else 1e-13) * o.M0
         if s not in m["save_steps"]:
             continue
         for key, a, b, fl, mk in (("phi", "phi", f"s{s}_phi", 0.0, fluid),
