@@ -30,6 +30,7 @@ extern "C" {
 enum lbm_precision { LBM_F32 = 0, LBM_F64 = 1, LBM_BF16 = 2, LBM_F16 = 3 };
 enum lbm_stab { LBM_STAB_FAKHARI = 0, LBM_STAB_LIANG = 1, LBM_STAB_NONE = 2 };
 enum lbm_frac_dtype { LBM_FRAC_NONE = 0, LBM_FRAC_F32 = 1, LBM_FRAC_F64 = 2 };
+enum lbm_interface { LBM_INTERFACE_AC = 0, LBM_INTERFACE_CH = 1 };
 
 enum lbm_status {
   LBM_OK = 0,
@@ -62,8 +63,7 @@ enum lbm_field {
 
 /* Constructor arguments: FEConfig (lbm/fe_config.py:27-150, after its
  * __post_init__ derivations) plus the AllenCahnSolver keyword arguments
- * (lbm/fe_ac_solver.py:54-58).  interface_mode / mu_drive / direct_wetting
- * are not part of this ABI (only the default Allen-Cahn path is built). */
+ * (lbm/fe_ac_solver.py:54-58). */
 typedef struct lbm_params {
   int nx, ny, nz;            /* nz == 1 selects D2Q9, otherwise D3Q19 */
   int precision;             /* enum lbm_precision */
@@ -79,6 +79,17 @@ typedef struct lbm_params {
   double ghost_scale;        /* <= 0: automatic (3 if |theta-90|>60 else 2) */
   int geometric_wetting;     /* 0/1 */
   double geo_amplification;  /* scalar alpha_geo (default 1.0) */
+  int direct_wetting;        /* 0/1: S12 relaxes towards the cubic wall-energy
+                                phi (lbm/fe_ac_solver.py:609-613,
+                                lbm/fe_wetting.py:313-416) */
+  int interface_mode;        /* enum lbm_interface: Allen-Cahn sharpening
+                                source, or the Cahn-Hilliard source
+                                M (grad mu - grad phi) tau_f/(tau_f-0.5)
+                                (lbm/fe_ac_solver.py:457-461) */
+  double mu_drive;           /* > 0 (Allen-Cahn only): adds
+                                mu_drive * M * grad mu to the source
+                                (lbm/fe_ac_solver.py:484-486).  Either form
+                                adds one pass per step (mu of every node). */
 } lbm_params;
 
 typedef struct lbm_solver lbm_solver;

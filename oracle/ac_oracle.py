@@ -247,6 +247,39 @@ def make_geometry(p: Params, solid, frac=None):
     return Geometry(solid, b, nrm, ww, near, dist, masks, nde, partial, frac, mode)
 
 
+def next_interior(solid, boundary, normal):
+    """Per boundary node, the node one rounded wall-normal step into the fluid
+    (coordinates clamped to the domain, not wrapped) and whether it is fluid
+    (lbm/fe_wetting.py:313-359).  Returns (bnd_coords, nsw_coords, nsw_fluid)."""
+    bc = np.argwhere(boundary)
+    shift = np.stack([np.rint(normal[d][tuple(bc.T)]) for d in range(solid.ndim)], 1)
+    nc = bc + shift.astype(np.int64)
+    for d in range(solid.ndim):
+        nc[:, d] = np.clip(nc[:, d], 0, solid.shape[d] - 1)
+    return bc, nc, ~solid[tuple(nc.T)]
+
+
+def cubic_boundary_phi(phi, bc, nc, nfluid, a):
+    """Boundary phi from the cubic wall energy (lbm/fe_wetting.py:369-416):
+    a phi_b^2 + (1 - a) phi_b - phi_nsw = 0 solved for phi_b in the run
+    dtype, where the next-interior node is fluid and 0.01 < phi_b,old < 0.99;
+    phi elsewhere."""
+    if len(bc) == 0 or abs(a) < 1e-10:
+        return phi
+    dt = phi.dtype.type
+    pn = phi[tuple(nc.T)]
+    pold = phi[tuple(bc.T)]
+    disc = dt((1.0 - a) ** 2) + dt(4.0 * a) * pn
+    disc = np.maximum(disc, dt(0.0))
+    pb = (dt(-(1.0 - a)) + np.sqrt(disc)) / dt(2.0 * a)
+    pb = np.clip(pb, dt(0.0), dt(1.0))
+    act = nfluid & (pold > dt(0.01)) & (pold < dt(0.99))
+    out = phi.copy()
+    sel = bc[act]
+    out[tuple(sel.T)] = pb[act]
+    return out
+
+
 # ---------------------------------------------------------------------------
 # 16-bit storage model (the build's precision extension; no reference
 # counterpart): f* is stored as f*_i(y) - w_i phi(y + e_i) and g* as is, both
@@ -280,8 +313,12 @@ class OracleSolver:
     def __init__(self, p: Params, dtype=np.float32, stab_mode='fakhari',
                  boundary_relax=0.3, ghost_scale=None, sharpen_reduce=0.8,
                  mu_wall_scale=1.0, geometric_wetting=False,
-                 geo_amplification=1.0, ac_scale=1.0, storage=None):
+                 geo_amplification=1.0, ac_scale=1.0, storage=None, direct_wetting=False,
+                 interface_mode='allen-cahn', mu_drive=0.0):
         self.p = p
+        self.direct_wetting = direct_wetting
+        self.interface_mode = interface_mode
+        self.mu_drive = mu_drive
         self.storage = storage          # None | 'bf16' | 'fp16' (fp32 compute)
         if storage is not None:
             dtype = np.float32
@@ -314,6 +351,7 @@ class OracleSolver:
     def set_solid(self, solid, frac=None):
         g = make_geometry(self.p, solid, frac)
         self.geom = g
+        self.nsw = next_interior(g.solid, g.is_boundary, g.wall_normal)
         self.solid = g.solid
         self.partial_eps = None
         if g.partial is not None and g.partial.any():
@@ -358,6 +396,17 @@ class OracleSolver:
                 grad[d] += (wi * self.ef[i, d]) * psi
             lap += (dt.type(2.0) * wi) * diff
         return grad, lap
+
+    def grad_only(self, a):
+        """Stand-alone gradient (lbm/fe_ac_solver.py:267-280) as the step
+        applies it to mu: ((w_i e_id) psi) / cs2 accumulated in direction
+        order, with the wetting ghost formula of phi evaluated on `a`."""
+        grad = np.zeros((self.D,) + a.shape, dtype=self.dt)
+        for i in range(1, self.Q):
+            psi = self._ghost(a, i, neighbor(a, self.e[i]))
+            for d in range(self.D):
+                grad[d] += ((self.w[i] * self.ef[i, d]) * psi) / CS2
+        return grad
 
     def lap_only(self, phi):
         """Stand-alone Laplacian used once by init_fields (fe_ac_solver.py:282-294):
@@ -545,7 +594,9 @@ class OracleSolver:
         mu = self.chem_pot(phi, lap)
         if self.wet is not None and self.geometric_wetting:
             grad = self.correct_gradient(grad)
-        st.update(phi=phi, grad=grad, lap=lap, mu=mu)
+        ch = self.interface_mode == 'cahn-hilliard'
+        gmu = self.grad_only(mu) if (ch or self.mu_drive > 0.0) else None
+        st.update(phi=phi, grad=grad, lap=lap, mu=mu, grad_mu=gmu)
         # S3
         P = self._qsum(self.g)
         # S4 (old u)
@@ -598,15 +649,24 @@ class OracleSolver:
             self.phi_u_prev = phiu.copy()
         dphiu = phiu - self.phi_u_prev
         self.phi_u_prev = phiu.copy()
-        gm = np.sqrt(self._usq(grad))
-        gm = np.maximum(gm, dt.type(1e-10))
-        nvec = grad / gm[None]
-        lam = ((4.0 * phi) * (1.0 - phi)) / p.xi
-        sh = ((CS2 * lam[None]) * nvec) * self.ac_scale
-        if self.wet is not None:
-            fac = 1.0 - self.sharpen_reduce * self.geom.wall_weight.astype(dt)
-            sh = sh * fac[None]
-        src = (dphiu + sh).astype(dt)
+        mob = CS2 * (self.tau_f - 0.5)
+        if ch:
+            # lbm/fe_ac_solver.py:457-461
+            guo = self.tau_f / (self.tau_f - 0.5)
+            src = (dphiu + (mob * (gmu - grad)) * guo).astype(dt)
+        else:
+            gm = np.sqrt(self._usq(grad))
+            gm = np.maximum(gm, dt.type(1e-10))
+            nvec = grad / gm[None]
+            lam = ((4.0 * phi) * (1.0 - phi)) / p.xi
+            sh = ((CS2 * lam[None]) * nvec) * self.ac_scale
+            if self.wet is not None:
+                fac = 1.0 - self.sharpen_reduce * self.geom.wall_weight.astype(dt)
+                sh = sh * fac[None]
+            src = (dphiu + sh).astype(dt)
+            if self.mu_drive > 0.0:
+                # lbm/fe_ac_solver.py:484-486
+                src = (src + (self.mu_drive * mob) * gmu).astype(dt)
         acoef = 1.0 - 0.5 / self.tau_f
         Fsrc = ((acoef * self._wv()) * self._edot(src)) / CS2
         # S8
@@ -660,9 +720,11 @@ class OracleSolver:
             phi_out = np.clip(phi * c, 0.0, 1.0).astype(dt)
             fn = (fn * c).astype(dt)
             st['c'] = c
-        # S12: boundary phi relax
+        # S12: boundary phi relax (target: the wall phi of the contact angle, or
+        # with direct_wetting the cubic wall-energy phi of each boundary node)
         if (self.wet is not None and self.boundary_relax > 0
-                and self.target is not None and abs(self.target - 0.5) > 0.05):
+                and (self.direct_wetting
+                     or (self.target is not None and abs(self.target - 0.5) > 0.05))):
             b = self.geom.is_boundary
             nw = self.geom.wall_normal.astype(dt)
             nz = np.abs(nw[D - 1])
@@ -670,7 +732,12 @@ class OracleSolver:
             top = (nz / nm) > 0.5
             act = b & top & (phi_out > 0.15) & (phi_out < 0.85)
             if act.any():
-                t = dt.type(self.target)
+                if self.direct_wetting:
+                    c = math.cos(math.radians(self.p.theta_eq))
+                    a = 6.0 * self.p.sigma * c / self.p.kappa
+                    t = cubic_boundary_phi(phi_out, *self.nsw, a)[act]
+                else:
+                    t = dt.type(self.target)
                 pn = np.clip(phi_out[act] + self.boundary_relax * (t - phi_out[act]),
                              0.0, 1.0).astype(dt)
                 ssum = self._qsum(fn)[act]

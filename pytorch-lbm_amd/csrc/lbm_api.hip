@@ -64,6 +64,9 @@ struct lbm_solver {
   uint8_t* solid = nullptr;
   uint32_t* wmask = nullptr;
   float* normal = nullptr;
+  int32_t* nsw = nullptr;     // direct wetting: next-interior node of each boundary node
+  void* mu = nullptr;         // Cahn-Hilliard / mu drive: mu of every node
+  int gmu = GMU_NONE;
   int16_t* surf = nullptr;
   double* partials = nullptr;
   void* mparts = nullptr;
@@ -194,8 +197,29 @@ template <typename T, int DIM> static Phys<T> make_phys(const lbm_solver* s) {
   }
   ph.body_on = body ? 1 : 0;
   ph.stab = p.stab_mode;
-  ph.s12_on = (s->wet != WET_NONE && p.boundary_relax > 0 && s->has_target &&
-               std::fabs(s->target - 0.5) > 0.05) ? 1 : 0;
+  ph.direct_on = (p.direct_wetting && s->wet != WET_NONE) ? 1 : 0;
+  ph.s12_on = (s->wet != WET_NONE && p.boundary_relax > 0 &&
+               (ph.direct_on || (s->has_target && std::fabs(s->target - 0.5) > 0.05))) ? 1 : 0;
+  if (ph.direct_on) {
+    // a = 6 sigma cos(theta) / kappa with math.radians' rounding
+    // (lbm/fe_wetting.py:361-364)
+    const double a = 6.0 * p.sigma * std::cos(p.theta_eq * (M_PI / 180.0)) / p.kappa;
+    ph.direct = std::fabs(a) >= 1e-10 ? 1 : 0;
+    ph.cub_4a = TT<T>(4.0 * a);
+    ph.cub_1ma2 = TT<T>((1.0 - a) * (1.0 - a));
+    ph.cub_m1a = TT<T>(-(1.0 - a));
+    ph.cub_2a = TT<T>(2.0 * a);
+    ph.lo01 = TT<T>(0.01);
+    ph.hi99 = TT<T>(0.99);
+    ph.nsw = s->nsw;
+  }
+  ph.gmu_mode = s->gmu;
+  if (s->gmu != GMU_NONE) {
+    const double mob = (1.0 / 3.0) * (tau_f - 0.5);
+    ph.mob = TT<T>(mob);
+    ph.guo = TT<T>(tau_f / (tau_f - 0.5));
+    ph.mu_drive_a = TT<T>(p.mu_drive * mob);
+  }
   ph.s12_r = TT<T>(p.boundary_relax);
   ph.s12_t = TT<T>(s->target);
   ph.thr_lo = TT<T>(0.15);
@@ -261,6 +285,8 @@ static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipS
   if (!(p.tau_l > 0.5) || !(p.tau_g > 0.5))
     return fail(LBM_EINVAL, "tau_l and tau_g must exceed 0.5");
   if (p.stab_mode < 0 || p.stab_mode > 2) return fail(LBM_EINVAL, "bad stab_mode");
+  if (p.interface_mode < 0 || p.interface_mode > 1)
+    return fail(LBM_EINVAL, "bad interface_mode");
   const long long N = (long long)p.nx * p.ny * p.nz;
   {
     // every device array is addressed with 32-bit byte offsets: the largest,
@@ -278,6 +304,8 @@ static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipS
   s->N = N;
   s->esz = p.precision == LBM_F64 ? 8 : 4;
   s->dsz = p.precision == LBM_F64 ? 8 : (p.precision == LBM_F32 ? 4 : 2);
+  s->gmu = p.interface_mode == LBM_INTERFACE_CH ? GMU_CH
+           : (p.mu_drive > 0.0 ? GMU_DRIVE : GMU_NONE);
   // wetting mode (lbm/fe_ac_solver.py:223-236, lbm/fe_wetting.py:85-115)
   if (p.theta_eq != 90.0 && p.kappa > 0) {
     s->wet = (p.theta_eq > 130.0 || p.theta_eq < 50.0) ? WET_LINEAR : WET_QUADRATIC;
@@ -329,6 +357,8 @@ static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipS
   ALLOC(s->mparts, (size_t)s->nblocks * sizeof(MeasRec));
   ALLOC(s->dout, 16 * 8);
   ALLOC(s->ctrl, sizeof(Ctrl));
+  if (p.direct_wetting && s->wet != WET_NONE) ALLOC(s->nsw, N * 4);
+  if (s->gmu != GMU_NONE) ALLOC(s->mu, N * es);
 #undef ALLOC
   if (e == hipSuccess && !async) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
@@ -347,7 +377,7 @@ int lbm_destroy(lbm_solver* s) {
   void* ptrs[] = {s->Fb[0], s->Fb[1], s->Gb[0], s->Gb[1], s->u, s->phi[0], s->phi[1],
                   s->eps, s->ampf, s->meta, s->bnd, s->dist[0], s->dist[1], s->solid,
                   s->wmask, s->normal, s->surf, s->partials, s->mparts, s->dout, s->ctrl,
-                  s->chunks, s->chunk_flag, s->keep};
+                  s->chunks, s->chunk_flag, s->keep, s->nsw, s->mu};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete s;
@@ -382,6 +412,7 @@ static int set_geometry_impl(lbm_solver* s, const void* frac, int fdt, hipStream
     k_meta<T, float, DIM, BS><<<nb, BS, 0, st>>>(g, s->solid, s->bnd, s->dist[0],
                                                  (const float*)frac, s->normal, s->meta,
                                                  (T*)s->eps, wet);
+  if (s->nsw) k_nsw<DIM, BS><<<nb, BS, 0, st>>>(g, s->bnd, s->solid, s->normal, s->nsw);
   const long long ncol = DIM == 3 ? (long long)g.nx * g.ny : g.nx;
   k_surface<DIM, BS><<<nblk(ncol), BS, 0, st>>>(g, s->solid, s->surf);
   // pass-B chunk lists: plain-fluid chunks, then the others (set-up only: one
@@ -594,7 +625,17 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     a.slot_prev = pslot;
     a.local = s->local ? 1 : 0;
     a.phiu_valid = s->phiu_valid ? 1 : 0;
-    if (!s->split) {
+    a.mu = (const T*)s->mu;
+    if (s->gmu != GMU_NONE) {
+      // grad mu needs mu at every neighbour: one more pass, then pass B in
+      // its grad-mu form (one launch, 256 threads)
+      k_mu<T, DIM, 256><<<nblk(s->N), 256, 0, st>>>(s->g, ph, (const T*)s->phi[nslot], s->meta,
+                                                    s->wmask, s->normal, (T*)s->mu);
+      if (s->local)
+        k_collide<T, S, DIM, 256, true, true><<<nblk(s->N), 256, 0, st>>>(a, ph);
+      else
+        k_collide<T, S, DIM, 256, false, true><<<nblk(s->N), 256, 0, st>>>(a, ph);
+    } else if (!s->split) {
 #define LAUNCH_B(B)                                               \
   if (s->local) k_collide<T, S, DIM, B, true><<<s->nbB, B, s->ldsB, st>>>(a, ph); \
   else k_collide<T, S, DIM, B, false><<<s->nbB, B, s->ldsB, st>>>(a, ph)
@@ -1380,6 +1421,8 @@ int lbm_batch_add(lbm_batch* b, const lbm_case_spec* sp, void* stream, int* case
   if (sp->p.precision != b->precision)
     return fail(LBM_EINVAL, "case precision differs from the batch precision");
   if (sp->p.precision == LBM_F64) return fail(LBM_EINVAL, "batch engine runs fp32 compute");
+  if (sp->p.interface_mode != LBM_INTERFACE_AC || sp->p.mu_drive > 0.0)
+    return fail(LBM_EINVAL, "batch engine runs the Allen-Cahn source without the mu drive");
   if (sp->steps < 0) return fail(LBM_EINVAL, "negative step target");
   if (sp->substrate < 0 || sp->substrate > 3) return fail(LBM_EINVAL, "bad substrate");
   const size_t need = case_bytes(sp->p);

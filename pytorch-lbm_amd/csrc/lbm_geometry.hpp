@@ -187,6 +187,33 @@ __global__ __launch_bounds__(BS) void k_meta(Grid g, const uint8_t* __restrict__
   eps[x] = e;
 }
 
+// Direct wetting: for each boundary node the node one rounded wall-normal step
+// into the fluid, coordinates clamped to the domain (not wrapped), or -1 when
+// that node is solid (lbm/fe_wetting.py:313-359).  Internal node indices.
+template <int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_nsw(Grid g, const uint8_t* __restrict__ bnd,
+                                            const uint8_t* __restrict__ solid,
+                                            const float* __restrict__ normal,
+                                            int32_t* __restrict__ nsw) {
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  int32_t r = -1;
+  if (bnd[x]) {
+    const Coord<DIM> c = decompose<DIM>(g, x);
+    int cc[3] = {c.x, c.y, c.z};
+    const int n[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      // torch.round: half to even, as rintf in the default rounding mode
+      const int v = cc[d] + (int)rintf(normal[(long long)d * g.N + x]);
+      cc[d] = v < 0 ? 0 : (v > n[d] - 1 ? n[d] - 1 : v);
+    }
+    const uint32_t q = int_index<DIM>(g, cc[0], cc[1], cc[2]);
+    if (!solid[q]) r = (int32_t)q;
+  }
+  nsw[x] = r;
+}
+
 // Topmost solid index per column (last axis), -1 if the column has none.
 // Columns are numbered as in the public arrays: x * ny + y (3D), x (2D).
 template <int DIM, int BS>

@@ -364,19 +364,56 @@ template <typename T> __device__ __forceinline__ Buf field_buf(const T* p, long 
   return Buf(p, (long long)comps * N * (long long)sizeof(T));
 }
 
+// phi at the next-interior node of a top boundary node x (start-of-step phi of
+// the previous step), or -1 where the cubic target does not apply; read only
+// under direct wetting.
+template <typename T>
+__device__ __forceinline__ T nsw_phi(const Phys<T>& ph, uint8_t m, const T* __restrict__ phi_prev,
+                                     uint32_t x) {
+  if (!ph.direct || !(m & M_TOP)) return T(-1);
+  const int32_t q = ph.nsw[x];
+  return q >= 0 ? phi_prev[q] : T(-1);
+}
+
+// S12 target of a top boundary node whose mass-corrected phi is pc: the wall
+// phi of the contact angle, or under direct wetting the cubic wall-energy phi
+// a phi_b^2 + (1 - a) phi_b = phi_nsw (lbm/fe_wetting.py:369-416) where the
+// next-interior node is fluid and 0.01 < pc < 0.99, else pc itself.
+// pns: raw phi of the next-interior node (-1: none), cfac: the mass factor.
+template <typename T>
+__device__ __forceinline__ T s12_target(const Phys<T>& ph, T pc, T pns, T cfac) {
+  if (!ph.direct_on) return ph.s12_t;
+  if (ph.direct && pns >= T(0) && pc > ph.lo01 && pc < ph.hi99) {
+    const T pn = clamp01(pns * cfac);
+    T disc = ph.cub_1ma2 + ph.cub_4a * pn;
+    disc = disc < T(0) ? T(0) : disc;
+    return clamp01((ph.cub_m1a + (T)sqrt(disc)) / ph.cub_2a);
+  }
+  return pc;
+}
+
+// Relaxed phi of a top boundary node (the S12 update), pc mass-corrected.
+template <typename T>
+__device__ __forceinline__ T s12_phi(const Phys<T>& ph, uint8_t m, T pc, T pns, T cfac) {
+  if (ph.s12_on && (m & M_TOP) && pc > ph.thr_lo && pc < ph.thr_hi)
+    return clamp01(pc + ph.s12_r * (s12_target<T>(ph, pc, pns, cfac) - pc));
+  return pc;
+}
+
 // Mass factor and boundary relax of the previous step applied to a pulled f
 // (lbm/fe_ac_solver.py:582-591, 594-636).  phi_prev is that step's
-// start-of-step phi; the relaxed phi is written to *phi_relaxed when given.
+// start-of-step phi at the node, pns the one at its next-interior node (see
+// nsw_phi); the relaxed phi is written to *phi_relaxed when given.
 template <typename T, int DIM>
 __device__ __forceinline__ void scale_f(const Phys<T>& ph, T (&f)[Lat<DIM>::Q], T cfac,
-                                        uint8_t m, T phi_prev, T* phi_relaxed) {
+                                        uint8_t m, T phi_prev, T* phi_relaxed, T pns = T(-1)) {
   constexpr int Q = Lat<DIM>::Q;
 #pragma unroll
   for (int i = 0; i < Q; ++i) f[i] = f[i] * cfac;
   if (ph.s12_on && (m & M_TOP)) {
     const T pc = clamp01(phi_prev * cfac);
     if (pc > ph.thr_lo && pc < ph.thr_hi) {
-      const T pn = clamp01(pc + ph.s12_r * (ph.s12_t - pc));
+      const T pn = clamp01(pc + ph.s12_r * (s12_target<T>(ph, pc, pns, cfac) - pc));
       const T tot = clamp_lo(qsum<Q>(f), T(1e-10));
       T s = pn / tot;
       s = s < T(0.5) ? T(0.5) : (s > T(2) ? T(2) : s);
@@ -537,7 +574,8 @@ __device__ __forceinline__ T phi_node(const Grid& g, const Phys<T>& ph, const S*
   pull<T, S, DIM, !std::is_same<T, S>::value, false, LOCAL>(bF, bF, x, nb, m, eps, bP, ph, f,
                                                             unused);
   if constexpr (!LOCAL)
-    scale_f<T, DIM>(ph, f, cprev, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr);
+    scale_f<T, DIM>(ph, f, cprev, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr,
+                    nsw_phi<T>(ph, m, phi_prev, x));
   T p = clamp01(qsum<Q>(f));
   if (m & M_SOLID) p = T(0);
   return p;
@@ -645,6 +683,60 @@ __device__ __forceinline__ T chem_pot(const Phys<T>& ph, T p, T lap, bool bnd) {
   return mu;
 }
 
+// Stand-alone gradient (lbm/fe_ac_solver.py:267-280) of the chemical
+// potential, as the step takes it: ((w_i e_id) psi) / cs2 accumulated in
+// direction order, psi = mu(x + e_i) or the wetting ghost formula of phi
+// evaluated on mu at wall directions.
+template <typename T, int DIM>
+__device__ __forceinline__ void grad_mu(const Phys<T>& ph, const T* __restrict__ mu_arr,
+                                        const Nbrs<DIM>& nb, T mc, uint32_t wm, const float* n32,
+                                        T* gmu) {
+  constexpr int Q = Lat<DIM>::Q;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) gmu[d] = T(0);
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    T psi = mu_arr[nb.i[i]];
+    if (wm & (1u << i)) {
+      float nde = 0.0f;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int e = E<DIM>(i, d);
+        if (e != 0) nde = nde + (e > 0 ? n32[d] : -n32[d]);
+        else nde = nde + 0.0f * n32[d];
+      }
+      if (ph.wet == WET_LINEAR) psi = mc - (T)(nde * ph.hk32);
+      else psi = mc + ((T)(nde * ph.pck32) * mc) * (T(1) - mc);
+    }
+    const T t = divk(ph.template w<DIM>(i) * psi, ph.cs2, ph.rcp_cs2);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      if (E<DIM>(i, d) != 0) gmu[d] = E<DIM>(i, d) > 0 ? gmu[d] + t : gmu[d] - t;
+  }
+}
+
+// Chemical potential of every node from the start-of-step phi (the input of
+// grad mu in the Cahn-Hilliard and mu-drive forms).
+template <typename T, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_mu(Grid g, Phys<T> ph, const T* __restrict__ phi,
+                                           const uint8_t* __restrict__ meta,
+                                           const uint32_t* __restrict__ wmask,
+                                           const float* __restrict__ normal,
+                                           T* __restrict__ mu) {
+  const uint32_t x = blockIdx.x * BS + threadIdx.x;
+  if (x >= g.N) return;
+  const uint8_t m = meta[x];
+  const bool bnd = (m & M_BOUNDARY) && ph.wet != WET_NONE;
+  float n32[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) n32[d] = bnd ? normal[(long long)d * g.N + x] : 0.0f;
+  T gr[DIM], lap;
+  const T p = phi[x];
+  stencil<T, DIM>(ph, g, phi, neighbours<DIM>(g, x, decompose<DIM>(g, x)), p,
+                  bnd ? wmask[x] : 0u, n32, gr, &lap, false);
+  mu[x] = chem_pot<T>(ph, p, lap, bnd);
+}
+
 // Geometric contact-angle correction at a wall-normal-carrying node
 // (lbm/fe_wetting.py:448-539); blend weight is 1 there (distance 0).
 template <typename T, int DIM>
@@ -702,11 +794,13 @@ template <typename T, typename S, int DIM> struct CollideArgs {
   int slot_new;
   int local;
   int phiu_valid;
+  const T* __restrict__ mu;  // chemical potential of every node (GMU kernels)
 };
 
 // FLUID: the caller guarantees a plain fluid node (no solid, partial, wall or
-// top flag), so every wall path compiles away.
-template <typename T, typename S, int DIM, bool LOCAL, bool FLUID = false>
+// top flag), so every wall path compiles away.  GMU: the interface source
+// uses grad mu (Cahn-Hilliard or the mu drive), from the mu of every node.
+template <typename T, typename S, int DIM, bool LOCAL, bool FLUID = false, bool GMU = false>
 __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
                                              const Phys<T>& ph, uint32_t x) {
   constexpr int Q = Lat<DIM>::Q;
@@ -755,6 +849,8 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   T gr[DIM], lap;
   stencil<T, DIM>(ph, g, a.phi_cur, nb, p, wm, n32, gr, &lap, false);
   const T mu = chem_pot<T>(ph, p, lap, bnd && ph.wet != WET_NONE);
+  T gmu[DIM];
+  if constexpr (GMU) grad_mu<T, DIM>(ph, a.mu, nb, mu, wm, n32, gmu);
   if (ph.corr_on && bnd) {
     const T an = a.ampf ? a.ampf[x] : T(0);
     correct_grad<T, DIM>(ph, gr, n32, an, a.ampf != nullptr);
@@ -855,6 +951,11 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
       T sh = ((ph.cs2 * lam) * nv) * ph.ac_scale;
       if (ph.wet != WET_NONE) sh = sh * sf;
       src[d] = dphiu + sh;
+      if constexpr (GMU) {
+        // lbm/fe_ac_solver.py:457-461 (Cahn-Hilliard) and :484-486 (mu drive)
+        if (ph.gmu_mode == GMU_CH) src[d] = dphiu + (ph.mob * (gmu[d] - gr[d])) * ph.guo;
+        else src[d] = src[d] + ph.mu_drive_a * gmu[d];
+      }
     }
   }
 
@@ -908,7 +1009,9 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   const Nbrs<DIM> nb2 = neighbours<DIM>(g, x2, decompose<DIM>(g, x2));
   T f[Q];
   pull<T, S, DIM, DEV, false, LOCAL, CP_FLD>(bFi, bFi, x, nb2, m, a.eps, bPp, ph, f, unused);
-  if constexpr (!LOCAL) scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot_prev], m, pprev, nullptr);
+  if constexpr (!LOCAL)
+    scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot_prev], m, pprev, nullptr,
+                    nsw_phi<T>(ph, m, a.phi_prev, x));
   T fsrc[Q];
   fsrc[0] = T(0);
 #pragma unroll
@@ -935,13 +1038,13 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
 #define LBM_COLLIDE_ATTR
 #endif
 
-template <typename T, typename S, int DIM, int BS, bool LOCAL>
+template <typename T, typename S, int DIM, int BS, bool LOCAL, bool GMU = false>
 __global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_collide(CollideArgs<T, S, DIM> a,
                                                                  Phys<T> ph) {
   if (blockIdx.x == 0) finalize_mass<BS>(a.partials, a.nparts, a.ctrl_w, a.slot_new);
   const uint32_t x = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (x >= a.g.N) return;
-  collide_node<T, S, DIM, LOCAL>(a, ph, x);
+  collide_node<T, S, DIM, LOCAL, false, GMU>(a, ph, x);
 }
 
 // Pass B over a list of 64-node chunks, one wave per chunk.  The chunks of a
@@ -1016,10 +1119,7 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
       T p = a.phi_cur[x];
       if (a.scaled) {
         const T cf = (T)a.ctrl->c[a.slot];
-        T pc = clamp01(p * cf);
-        if (ph.s12_on && (m & M_TOP) && pc > ph.thr_lo && pc < ph.thr_hi)
-          pc = clamp01(pc + ph.s12_r * (ph.s12_t - pc));
-        p = pc;
+        p = s12_phi<T>(ph, m, clamp01(p * cf), nsw_phi<T>(ph, m, a.phi_cur, x), cf);
       }
       a.out[ex] = p;
       break;
@@ -1063,7 +1163,7 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
       }
       if (a.kind == OBS_F && a.scaled && !a.local)
         scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot], m, (m & M_TOP) ? a.phi_cur[x] : T(0),
-                        nullptr);
+                        nullptr, nsw_phi<T>(ph, m, a.phi_cur, x));
 #pragma unroll
       for (int i = 0; i < Q; ++i) a.out[(long long)i * N + ex] = f[i];
       break;
@@ -1153,7 +1253,8 @@ __global__ __launch_bounds__(BS) void k_inject(InjectArgs<T, S, DIM> a, Phys<T> 
       const T cf = (T)a.ctrl->c[a.slot];
       const T praw = p;
       p = clamp01(praw * cf);
-      scale_f<T, DIM>(ph, f, cf, m, (m & M_TOP) ? praw : T(0), &p);
+      scale_f<T, DIM>(ph, f, cf, m, (m & M_TOP) ? praw : T(0), &p,
+                      nsw_phi<T>(ph, m, a.pref, x));
     }
     const T C = a.C[x];
     const bool core = C > T(0.1);

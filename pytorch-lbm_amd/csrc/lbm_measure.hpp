@@ -20,12 +20,11 @@ struct MeasRec {
 };
 
 template <typename T, int DIM>
-__device__ __forceinline__ T phi_observed(const Phys<T>& ph, T p, uint8_t m, T cf, int scaled) {
+__device__ __forceinline__ T phi_observed(const Phys<T>& ph, const T* __restrict__ phi,
+                                          uint32_t x, uint8_t m, T cf, int scaled) {
+  const T p = phi[x];
   if (!scaled) return p;
-  T pc = clamp01(p * cf);
-  if (ph.s12_on && (m & M_TOP) && pc > ph.thr_lo && pc < ph.thr_hi)
-    pc = clamp01(pc + ph.s12_r * (ph.s12_t - pc));
-  return pc;
+  return s12_phi<T>(ph, m, clamp01(p * cf), nsw_phi<T>(ph, m, phi, x), cf);
 }
 
 template <int BS>
@@ -73,7 +72,7 @@ __global__ __launch_bounds__(BS) void k_measure(Grid g, Phys<T> ph, const T* __r
     const uint32_t x = blockIdx.x * BS + threadIdx.x + (uint32_t)k * stride;
     if (x >= g.N) break;
     const uint8_t m = meta[x];
-    T p = phi_observed<T, DIM>(ph, phi[x], m, cf, scaled);
+    T p = phi_observed<T, DIM>(ph, phi, x, m, cf, scaled);
     if (p != p) r.nan = 1;
     if (m & M_SOLID) p = T(0);
     r.vol += (double)p;
@@ -159,7 +158,7 @@ __global__ __launch_bounds__(BS) void k_contact(Grid g, Phys<T> ph, const T* __r
       const long long x = int_index<DIM>(g, (int)(col / g.ny), (int)(col % g.ny), z);
       const uint8_t m = meta[x];
       const T cf = scaled ? (T)ctrl->c[slot] : T(1);
-      const T p = phi_observed<T, DIM>(ph, phi[x], m, cf, scaled);
+      const T p = phi_observed<T, DIM>(ph, phi, x, m, cf, scaled);
       if (!(m & M_SOLID) && p > T(0.5)) {
         const double dx = (double)(col / g.ny) - cx, dy = (double)(col % g.ny) - cy;
         r2 = dx * dx + dy * dy;

@@ -10,6 +10,8 @@
 namespace lbm {
 
 enum WetMode : int { WET_NONE = 0, WET_LINEAR = 1, WET_QUADRATIC = 2 };
+// interface sources that use grad mu
+enum GmuMode : int { GMU_NONE = 0, GMU_CH = 1, GMU_DRIVE = 2 };
 
 // Direction-dependent constants are stored once per weight class (rest, axis,
 // diagonal): with 19 per-direction copies the kernels ran out of scalar
@@ -41,6 +43,16 @@ template <typename T> struct Phys {
   int s12_on;
   T s12_r, s12_t;
   T thr_lo, thr_hi;   // 0.15, 0.85
+  // direct wetting: S12 target = cubic wall-energy phi of each top boundary
+  // node, from the phi of its next-interior node nsw[x] (-1: none / solid)
+  int direct;         // direct_wetting and a != 0
+  int direct_on;      // direct_wetting (S12 runs even without a cubic target)
+  T cub_4a, cub_1ma2, cub_m1a, cub_2a;   // 4a, (1-a)^2, -(1-a), 2a in T
+  T lo01, hi99;       // 0.01, 0.99
+  const int32_t* nsw;
+  int gmu_mode;       // GmuMode
+  T mob, guo;         // cs2 (tau_f - 0.5), tau_f / (tau_f - 0.5)
+  T mu_drive_a;       // mu_drive * mobility
 
   template <int DIM> __host__ __device__ T w(int i) const { return wc[wclass<DIM>(i)]; }
 };

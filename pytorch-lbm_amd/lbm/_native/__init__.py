@@ -48,6 +48,9 @@ class LbmParams(ctypes.Structure):
         ("ghost_scale", ctypes.c_double),
         ("geometric_wetting", ctypes.c_int),
         ("geo_amplification", ctypes.c_double),
+        ("direct_wetting", ctypes.c_int),
+        ("interface_mode", ctypes.c_int),
+        ("mu_drive", ctypes.c_double),
     ]
 
 

@@ -9,9 +9,10 @@ stream and materialises the observable attributes (f, g, phi, rho, u, P, p,
 mu_phi) on first access after each step.  There is no CPU path: the solver
 requires a HIP device and fails loudly otherwise.
 
-Options outside the default Allen-Cahn path of the reference
-(interface_mode='cahn-hilliard', mu_drive > 0, direct_wetting) raise
-NotImplementedError instead of silently running something else.
+Every constructor option of the reference is built: interface_mode
+('allen-cahn' | 'cahn-hilliard'), mu_drive, direct_wetting, stab_mode,
+geometric_wetting with scalar or per-node amplification; ch_wall_blend is
+accepted and, as in the reference, only stored.
 """
 import ctypes
 import math
@@ -26,6 +27,7 @@ from .fe_wetting import ConningtonLeeWetting
 from .lattice import D2Q9, D3Q19
 
 _STAB = {"fakhari": 0, "liang": 1, "none": 2}
+_INTERFACE = {"allen-cahn": 0, "cahn-hilliard": 1}
 _STORAGE = {None: None, "fp32": nat.LBM_F32, "fp64": nat.LBM_F64,
             "bf16": nat.LBM_BF16, "fp16": nat.LBM_F16}
 
@@ -44,12 +46,8 @@ class AllenCahnSolver:
                  ghost_scale=None, sharpen_reduce=0.8, mu_wall_scale=1.0,
                  geometric_wetting=False, direct_wetting=False, ch_wall_blend=0.0,
                  geo_amplification=1.0, ac_scale=1.0, storage=None, verbose=False):
-        if interface_mode != "allen-cahn":
-            raise NotImplementedError("only interface_mode='allen-cahn' is built")
-        if mu_drive != 0.0:
-            raise NotImplementedError("mu_drive > 0 (needs grad mu) is not built")
-        if direct_wetting:
-            raise NotImplementedError("direct_wetting (cubic wall energy) is not built")
+        if interface_mode not in _INTERFACE:
+            raise ValueError(f"unknown interface_mode {interface_mode!r}")
         if dtype not in (torch.float32, torch.float64):
             raise ValueError("dtype must be torch.float32 or torch.float64")
         self.config = cfg = config
@@ -112,6 +110,9 @@ class AllenCahnSolver:
         p.ac_scale = float(ac_scale)
         p.ghost_scale = float(ghost_scale) if ghost_scale is not None else 0.0
         p.geometric_wetting = 1 if geometric_wetting else 0
+        p.direct_wetting = 1 if direct_wetting else 0
+        p.interface_mode = _INTERFACE[interface_mode]
+        p.mu_drive = float(mu_drive)
         amp_tensor = torch.is_tensor(geo_amplification)
         p.geo_amplification = 1.0 if amp_tensor else float(geo_amplification)
         self._params = p

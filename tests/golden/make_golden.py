@@ -20,7 +20,9 @@ routines and invariants in tests/test_oracle_*.py):
     step-1 phi (lbm/fe_ac_solver.py:296-320, lbm/fe_wetting.py:448-539);
   * the stab_mode variants and the body force of the force stage
     (lbm/fe_ac_solver.py:415-439);
-  * ``inject_droplet`` between two steps (lbm/fe_ac_solver.py:645-732).
+  * ``inject_droplet`` between two steps (lbm/fe_ac_solver.py:645-732);
+  * direct wetting (lbm/fe_ac_solver.py:609-613, lbm/fe_wetting.py:313-416),
+    the Cahn-Hilliard source and the mu drive (lbm/fe_ac_solver.py:457-486).
 
 Usage (in the build container only):  python tests/golden/make_golden.py
 """
@@ -122,6 +124,35 @@ CASES = {
         kw=dict(geo_amplification=1.5, geometric_wetting=True, stab_mode="none"),
         dtype="float64", center=(16.0, 12.0), radius=7.0,
         u=(0.0, -0.02), frac=True, tau_h=None, fg_steps=(1,), g_force=(0.0, -1e-5)),
+    # direct wetting: S12 relaxes towards the cubic wall-energy boundary phi
+    "ridge120_direct_2d_f64": dict(
+        grid=(32, 32, 1), sub="ridge", R_star=1.0, R_d=5.0, theta=120.0,
+        kw=dict(geo_amplification=1.5, geometric_wetting=True, direct_wetting=True),
+        dtype="float64", center=(16.0, 11.5), radius=5.0,
+        u=(0.0, -0.04), frac=True, tau_h=None, fg_steps=(1, 2)),
+    "ridge144_direct_f32": dict(
+        grid=(20, 20, 16), sub="ridge", R_star=1.0, R_d=5.0, theta=144.0,
+        kw=dict(geo_amplification=1.5, geometric_wetting=True, direct_wetting=True),
+        dtype="float32", center=(10.0, 10.0, 10.0), radius=4.0,
+        u=(0.0, 0.0, -0.05), frac=True, tau_h=None, fg_steps=(1,)),
+    # Cahn-Hilliard interface source and the mu drive (both use grad mu)
+    "ridge120_ch_2d_f64": dict(
+        grid=(32, 32, 1), sub="ridge", R_star=1.0, R_d=5.0, theta=120.0,
+        kw=dict(geo_amplification=1.5, geometric_wetting=True,
+                interface_mode="cahn-hilliard"),
+        dtype="float64", center=(16.0, 11.5), radius=5.0,
+        u=(0.0, -0.04), frac=True, tau_h=None, fg_steps=(1,)),
+    "flat144_mudrive_2d_f64": dict(
+        grid=(32, 32, 1), sub="flat", R_star=None, R_d=None, theta=144.0,
+        kw=dict(geo_amplification=1.5, geometric_wetting=True, mu_drive=0.5),
+        dtype="float64", center=(16.0, 8.0), radius=6.0,
+        u=(0.0, -0.02), frac=True, tau_h=None, fg_steps=(1,)),
+    "ridge162_ch_f32": dict(
+        grid=(20, 20, 16), sub="ridge", R_star=1.0, R_d=5.0, theta=162.0,
+        kw=dict(geo_amplification=1.5, geometric_wetting=True,
+                interface_mode="cahn-hilliard"),
+        dtype="float32", center=(10.0, 10.0, 10.0), radius=4.0,
+        u=(0.0, 0.0, -0.05), frac=True, tau_h=None, fg_steps=(1,)),
     # a second drop injected after step 3 (3D, fp64) and after step 2 (2D, fp32)
     "ridge162_inject_f64": dict(
         grid=(20, 20, 16), sub="ridge", R_star=1.0, R_d=5.0, theta=162.0,
