@@ -281,6 +281,13 @@ int lbm_batch_remove(lbm_batch* b, int case_id);
 /* The solver handle of a case, for lbm_get_field / lbm_measure / ... */
 int lbm_batch_solver(lbm_batch* b, int case_id, lbm_solver** s);
 
+/* Restore a case from an lbm_save_state image of its solver (taken with
+ * lbm_batch_solver + lbm_save_state): the case resumes bit for bit at the
+ * image's step.  For crash-safe sweep queues (the reference's sweep resume
+ * only skips finished cases, scripts/run_pending_sweep.py:207-240).
+ * Synchronises `stream`; a captured batch graph must be captured again. */
+int lbm_batch_load_state(lbm_batch* b, int case_id, const void* src, void* stream);
+
 /* Steps done and step target of a case. */
 int lbm_batch_status(lbm_batch* b, int case_id, long long* steps_done, long long* steps_target);
 

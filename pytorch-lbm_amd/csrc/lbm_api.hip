@@ -1470,6 +1470,17 @@ int lbm_batch_solver(lbm_batch* b, int id, lbm_solver** s) {
   return LBM_OK;
 }
 
+int lbm_batch_load_state(lbm_batch* b, int id, const void* src, void* stream) {
+  if (!b || id < 0 || id >= (int)b->cs.size() || !b->cs[id]) return fail(LBM_EINVAL, "bad case id");
+  lbm_solver* s = b->cs[id];
+  int rc = lbm_load_state(s, src, stream);
+  if (rc) return rc;
+  if (s->k > b->k_end[id]) return fail(LBM_EINVAL, "state is past the case's step target");
+  // the per-case descriptors (step counter, buffer parity, form) are rebuilt
+  b->dirty = true;
+  return LBM_OK;
+}
+
 int lbm_batch_status(lbm_batch* b, int id, long long* done, long long* target) {
   if (!b || id < 0 || id >= (int)b->cs.size() || !b->cs[id]) return fail(LBM_EINVAL, "bad case id");
   if (done) *done = b->cs[id]->k;

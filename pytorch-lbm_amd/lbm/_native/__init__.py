@@ -106,6 +106,7 @@ _SIGS = {
                                      ctypes.POINTER(ctypes.c_int)]),
     "lbm_batch_remove": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbm_batch_solver": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "lbm_batch_load_state": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "lbm_batch_status": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
                                         ctypes.POINTER(ctypes.c_longlong)]),
     "lbm_batch_bytes_used": (ctypes.c_size_t, [_vp]),

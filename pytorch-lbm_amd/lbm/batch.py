@@ -183,6 +183,22 @@ class BatchEngine:
                   "lbm_batch_solver")
         return s
 
+    def save_case(self, cid):
+        """Host copy (uint8 tensor) of case cid's whole state; synchronises."""
+        s = self._solver(cid)
+        n = int(nat.lib.lbm_state_bytes(s))
+        buf = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+        with torch.cuda.device(self.device):
+            nat.check(nat.lib.lbm_save_state(s, nat.ptr(buf), self._stream()), "lbm_save_state")
+        return buf
+
+    def load_case(self, cid, state):
+        """Resume case cid from a save_case() image (same spec, same storage)."""
+        buf = state if state.is_pinned() or state.is_cuda else state.pin_memory()
+        with torch.cuda.device(self.device):
+            nat.check(nat.lib.lbm_batch_load_state(self._h, int(cid), nat.ptr(buf),
+                                                   self._stream()), "lbm_batch_load_state")
+
     def field(self, cid, name):
         """A copy of an observable field of case cid (phi, u, f, ...)."""
         spec = self.cases[cid]

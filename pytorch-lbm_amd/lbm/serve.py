@@ -12,9 +12,19 @@ completion.
 
 Every case keeps its own buffers, reduction tree and step counter, so a
 request's result does not depend on what else shares the launches.
+
+Crash safety (the reference resumes a sweep by skipping labels already in its
+results file, scripts/run_pending_sweep.py:207-240, 271-279): a ResultStore
+keeps that label-keyed JSON list, rewritten atomically after every completed
+request, and optional checkpoints hold the full device state of every
+resident case; a restarted server skips finished labels and resumes
+unfinished ones from their last checkpoint, with results identical to an
+uninterrupted run.
 """
 from __future__ import annotations
 
+import json
+import os
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -76,6 +86,36 @@ def poisson_requests(seed=5, rate=4.0, duration=20.0, steps=(500, 3000),
     return out
 
 
+class ResultStore:
+    """Label-keyed results file: a JSON list of dicts with a "label" key.
+    Labels present are done; add() appends a result and rewrites the file
+    through a temporary file and an atomic rename, so a crash leaves either
+    the old or the new list."""
+
+    def __init__(self, path):
+        self.path = path
+        self.results = []
+        if os.path.exists(path):
+            with open(path) as f:
+                self.results = json.load(f)
+        self.labels = {r["label"] for r in self.results}
+
+    def done(self, label):
+        return label in self.labels
+
+    def add(self, result):
+        if result["label"] in self.labels:
+            raise ValueError(f"duplicate label {result['label']!r}")
+        self.results.append(result)
+        self.labels.add(result["label"])
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.results, f, indent=2, default=str)
+            f.flush()
+            os.fsync(f.fileno())
+        os.replace(tmp, self.path)
+
+
 class SweepServer:
     """Admission by HBM footprint and execution slots; continuous batching.
 
@@ -85,7 +125,8 @@ class SweepServer:
     that no resident case overshoots its target)."""
 
     def __init__(self, storage="bf16", budget_bytes=0, max_active=32, chunk=50,
-                 device=None):
+                 device=None, store: Optional[ResultStore] = None,
+                 checkpoint_dir: Optional[str] = None, checkpoint_every=0):
         self.engine = BatchEngine(storage, budget_bytes=budget_bytes, device=device)
         self.storage = storage
         self.budget = int(budget_bytes)
@@ -93,6 +134,33 @@ class SweepServer:
         self.chunk = int(chunk)
         self.active: Dict[int, Request] = {}
         self.done: List[Request] = []
+        self.skipped: List[Request] = []
+        self.store = store
+        self.ckpt_dir = checkpoint_dir
+        self.ckpt_every = int(checkpoint_every)
+        if checkpoint_dir:
+            os.makedirs(checkpoint_dir, exist_ok=True)
+
+    # -- checkpoints of resident cases ----------------------------------------
+    def _ckpt_path(self, req):
+        return os.path.join(self.ckpt_dir, f"{req.spec.tag}.state.npy")
+
+    def _checkpoint(self):
+        import numpy as np
+        for cid, req in self.active.items():
+            path = self._ckpt_path(req)
+            tmp = path + ".tmp.npy"
+            np.save(tmp, self.engine.save_case(cid).numpy())
+            os.replace(tmp, path)
+
+    def _resume(self, req):
+        if not self.ckpt_dir or not os.path.exists(self._ckpt_path(req)):
+            return False
+        import numpy as np
+        import torch
+        state = torch.from_numpy(np.load(self._ckpt_path(req)))
+        self.engine.load_case(req.case_id, state)
+        return True
 
     def _fits(self, req):
         if len(self.active) >= self.max_active:
@@ -113,13 +181,33 @@ class SweepServer:
         self.engine.remove(cid)
         del self.active[cid]
         self.done.append(req)
+        if self.store is not None:
+            rec = {k: v for k, v in req.result.items() if k != "phi"}
+            rec["label"] = req.spec.tag
+            rec["steps"] = req.spec.steps
+            self.store.add(rec)
+        if self.ckpt_dir and os.path.exists(self._ckpt_path(req)):
+            os.remove(self._ckpt_path(req))
 
-    def run(self, requests, keep_fields=False, on_event=None):
+    def run(self, requests, keep_fields=False, on_event=None, max_chunks=None):
         """Serve `requests` (sorted by arrival) in real time; returns them
-        with admission / completion times and results."""
+        with admission / completion times and results.  Requests whose label
+        the store already holds are skipped; max_chunks stops the loop early
+        (a preemption), leaving resident cases to their last checkpoint."""
         import torch
         self.keep_fields = keep_fields
-        pending = deque(sorted(requests, key=lambda r: r.t_arrival))
+        if self.store is not None or self.ckpt_dir:
+            tags = [r.spec.tag for r in requests]
+            if "" in tags or len(set(tags)) != len(tags):
+                raise ValueError("a result store or checkpoints need unique, non-empty tags")
+        todo = []
+        for r in requests:
+            if self.store is not None and self.store.done(r.spec.tag):
+                self.skipped.append(r)
+            else:
+                todo.append(r)
+        pending = deque(sorted(todo, key=lambda r: r.t_arrival))
+        chunks = 0
         waiting: deque = deque()
         t0 = time.perf_counter()
         self.lattice_updates = 0
@@ -133,8 +221,9 @@ class SweepServer:
                 req.case_id = self.engine.add(req.spec)
                 req.t_admit = time.perf_counter() - t0
                 self.active[req.case_id] = req
+                resumed = self._resume(req)
                 if on_event:
-                    on_event("admit", req)
+                    on_event("resume" if resumed else "admit", req)
             if not self.active:
                 if pending:
                     time.sleep(max(0.0, min(0.05, pending[0].t_arrival - now)))
@@ -142,7 +231,10 @@ class SweepServer:
             left = min(r.spec.steps - self.engine.status(cid)[0]
                        for cid, r in self.active.items())
             n = max(1, min(self.chunk, left))
+            if max_chunks is not None and chunks >= max_chunks:
+                break
             self.engine.step(n)
+            chunks += 1
             self.lattice_updates += n * sum(r.spec.nodes for r in self.active.values())
             torch.cuda.current_stream(self.engine.device).synchronize()
             now = time.perf_counter() - t0
@@ -152,5 +244,7 @@ class SweepServer:
                     self._finish(req, now)
                     if on_event:
                         on_event("done", req)
+            if self.ckpt_dir and self.ckpt_every > 0 and chunks % self.ckpt_every == 0:
+                self._checkpoint()
         self.wall = time.perf_counter() - t0
         return self.done

@@ -70,3 +70,40 @@ def test_admission_respects_hbm_budget():
     done = srv.run(reqs, on_event=lambda k, r: peak.append(srv.engine.bytes_used()))
     assert len(done) == 6
     assert max(peak) <= budget
+
+
+def test_preempted_server_resumes_identically(tmp_path):
+    """A server stopped mid-run (preemption) and restarted with the same result
+    store and checkpoint directory skips finished labels, resumes resident
+    cases from their checkpoints, and ends with the results of an
+    uninterrupted run, bit for bit."""
+    need_gpu()
+    print("[gpu] serve preempt/resume", flush=True)
+    from lbm.serve import ResultStore, SweepServer
+    ref = {r.spec.tag: r for r in SweepServer("bf16", max_active=3, chunk=8).run(
+        small_stream(), keep_fields=True)}
+    store_path = str(tmp_path / "results.json")
+    ckpt = str(tmp_path / "ckpt")
+    first = SweepServer("bf16", max_active=3, chunk=8, store=ResultStore(store_path),
+                        checkpoint_dir=ckpt, checkpoint_every=2)
+    first.run(small_stream(), max_chunks=7)
+    n_first = len(first.done)
+    assert 0 < n_first < 6                        # stopped with work in flight
+    events = []
+    second = SweepServer("bf16", max_active=3, chunk=8, store=ResultStore(store_path),
+                         checkpoint_dir=ckpt, checkpoint_every=2)
+    second.run(small_stream(), keep_fields=True,
+               on_event=lambda k, r: events.append((k, r.spec.tag)))
+    assert len(second.skipped) == n_first
+    assert any(k == "resume" for k, _ in events)  # resumed from a checkpoint
+    final = ResultStore(store_path)
+    assert sorted(final.labels) == sorted(ref)
+    for rec in final.results:
+        want = ref[rec["label"]].result
+        for k in ("Dx", "Dy", "Dz", "z_com", "volume", "contact_radius"):
+            assert rec[k] == want[k], (rec["label"], k)
+    for req in second.done:
+        np.testing.assert_array_equal(req.result["phi"].numpy(),
+                                      ref[req.spec.tag].result["phi"].numpy())
+    import os
+    assert not [f for f in os.listdir(ckpt) if f.endswith(".npy")]
