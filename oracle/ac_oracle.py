@@ -15,7 +15,12 @@ It restates, in plain numpy, the behaviour of JunjunQing/pytorch-lbm:
   stencil                  lbm/fe_ac_solver.py:296-320 + fe_wetting.py:418-446
   chemical potential       lbm/fe_ac_solver.py:326-340
   gradient correction      lbm/fe_wetting.py:448-539
-  step (S1..S13)           lbm/fe_ac_solver.py:380-639
+  step (S1..S13)           lbm/fe_ac_solver.py:380-639, including the
+                           stab_mode variants and body force (:415-439), the
+                           Cahn-Hilliard source and mu drive (:457-486, with
+                           the gradient of mu, :267-280) and direct wetting
+                           (:609-613, fe_wetting.py:313-416)
+  inject_droplet           lbm/fe_ac_solver.py:645-732
   diagnostics              lbm/fe_measure.py:11-241
 
 Arithmetic is done in the run dtype (float32 or float64) with the same
@@ -28,12 +33,15 @@ weight) are kept in float32 here too.
 
 Parity pins (tests/test_oracle_*.py):
   * golden vectors written by tests/golden/make_golden.py from the reference
-    on CPU: geometry masks bit-exact, operator outputs, 60-step trajectories;
+    on CPU: geometry masks bit-exact, operator outputs, 60-step trajectories
+    (every solver option above, and the state right after an injection);
   * closed forms: lattice moment identities, derived constants, MMS gradient
     error numbers published with the reference (N=32: 1.33e-3, N=128: 2.09e-5);
   * np.roll / np.gradient library semantics for streaming and normals;
   * invariants: mass after every step, constant fields, correction no-op where
-    the wall normal is zero.
+    the wall normal is zero; for inject_droplet, a drop injected into an empty
+    domain (phi = C on the core, equilibrium moments), untouched nodes outside
+    the core, idempotence of max(phi, C).
 Parity unpinned: none of the functions below; the fp32 summation order inside
 torch reductions (Sum over directions, einsum) is not reproducible and is
 covered by tolerance, not bit-exactness.
