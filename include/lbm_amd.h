@@ -229,6 +229,13 @@ int lbm_load_state(lbm_solver* s, const void* src, void* stream);
  * Synchronises `stream`. */
 int lbm_measure(lbm_solver* s, double* out, void* stream);
 
+/* lbm_measure without synchronising: the same 10 doubles are written to
+ * device memory out_dev in stream order, so a driver can watch the NaN flag
+ * (out_dev[9]) and the extents between launches without blocking the stream
+ * (the reference checks nothing inside step() and copies phi to the host
+ * every 50 steps, scripts/run_thesis_sweep.py:185-206).  Graph-capturable. */
+int lbm_measure_async(lbm_solver* s, double* out_dev, void* stream);
+
 /* Contact-line radius: max horizontal distance from (cx, cy) of nodes with
  * phi > 0.5 in the first non-solid layer above the substrate at each column
  * (3D only).  Synchronises `stream`. */

@@ -1226,15 +1226,19 @@ int lbm_load_state(lbm_solver* s, const void* src, void* stream) {
 }
 
 
+// dev_out: 10 doubles of device memory receive the result (no copy back, no
+// synchronisation); otherwise the solver's own buffer is copied to `out`.
 template <typename T, int DIM>
-static int measure_impl(lbm_solver* s, double* out, hipStream_t st) {
+static int measure_impl(lbm_solver* s, double* out, hipStream_t st, bool dev_out = false) {
   const Phys<T> ph = make_phys<T, DIM>(s);
   const unsigned int nb = (unsigned int)((s->N + (long long)BS * kMeasPer - 1) / ((long long)BS * kMeasPer));
   k_measure<T, DIM, BS><<<nb, BS, 0, st>>>(
       s->g, ph, (const T*)s->phi[s->k & 1], s->meta, s->ctrl, (int)(s->k & 1),
       s->local ? 0 : 1, (MeasRec*)s->mparts);
-  k_measure_final<DIM, BS><<<1, BS, 0, st>>>((const MeasRec*)s->mparts, nb, s->dout);
+  k_measure_final<DIM, BS><<<1, BS, 0, st>>>((const MeasRec*)s->mparts, nb,
+                                             dev_out ? out : s->dout);
   HIPCHK(hipGetLastError());
+  if (dev_out) return LBM_OK;
   HIPCHK(hipMemcpyAsync(out, s->dout, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return LBM_OK;
@@ -1244,6 +1248,14 @@ int lbm_measure(lbm_solver* s, double* out, void* stream) {
   if (!s || !out) return fail(LBM_EINVAL, "null argument");
   if (!s->inited) return fail(LBM_ESTATE, "fields not initialised");
 #define CALL(T, D) measure_impl<T, D>(s, out, (hipStream_t)stream)
+  return DISPATCH(s, CALL);
+#undef CALL
+}
+
+int lbm_measure_async(lbm_solver* s, double* out_dev, void* stream) {
+  if (!s || !out_dev) return fail(LBM_EINVAL, "null argument");
+  if (!s->inited) return fail(LBM_ESTATE, "fields not initialised");
+#define CALL(T, D) measure_impl<T, D>(s, out_dev, (hipStream_t)stream, true)
   return DISPATCH(s, CALL);
 #undef CALL
 }

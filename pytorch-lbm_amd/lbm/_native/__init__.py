@@ -85,6 +85,7 @@ _SIGS = {
     "lbm_step_count": (ctypes.c_longlong, [_vp]),
     "lbm_inject_droplet": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_double), _vp]),
     "lbm_measure": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), _vp]),
+    "lbm_measure_async": (ctypes.c_int, [_vp, _vp, _vp]),
     "lbm_contact_radius": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
                                           ctypes.POINTER(ctypes.c_double), _vp]),
     "lbm_last_error": (ctypes.c_char_p, []),

@@ -402,6 +402,17 @@ class AllenCahnSolver:
                 "phi_sum", "nan")
         return {k: float(v) for k, v in zip(keys, out)}
 
+    def measure_async(self, out=None):
+        """The diagnostics of measure() written to a device tensor of 10
+        float64 (Dx, Dy, Dz, k, volume, C_max, z_com, n_liquid, phi_sum,
+        nan_flag) in stream order, without synchronising."""
+        if out is None:
+            out = torch.empty(10, dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            nat.check(nat.lib.lbm_measure_async(self._h, nat.ptr(out), self._stream()),
+                      "lbm_measure_async")
+        return out
+
     def contact_radius(self, cx=None, cy=None):
         """Max distance from (cx, cy) of liquid nodes on the first fluid layer
         above the substrate (3D)."""

@@ -40,3 +40,9 @@ def test_measure_and_contact_radius(name):
         if phi.ndim == 3:
             cx, cy = phi.shape[0] / 2.0, phi.shape[1] / 2.0
             assert s.contact_radius(cx, cy) == contact_radius(phi, solid, cx, cy)
+        # the non-blocking form writes the same record to device memory
+        dev = s.measure_async()
+        keys = ("Dx", "Dy", "Dz", "k", "volume", "C_max", "z_com", "n_liquid", "phi_sum", "nan")
+        assert dev.is_cuda
+        for k, v in zip(keys, dev.cpu().tolist()):
+            assert v == got[k], (k, v, got[k])
