@@ -1,2 +1,2 @@
-"""Output formats of the reference's drivers (io_utils/vtk_writer.py; the
-sweep's .npz snapshots, scripts/run_thesis_sweep.py:218-241)."""
+from .vtk_writer import write_vtk
+from .monitor import Monitor

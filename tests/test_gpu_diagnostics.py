@@ -46,3 +46,32 @@ def test_measure_and_contact_radius(name):
         assert dev.is_cuda
         for k, v in zip(keys, dev.cpu().tolist()):
             assert v == got[k], (k, v, got[k])
+
+
+def test_reference_measure_helpers_on_the_solver():
+    """The reference-named host helpers read the HIP solver's observables:
+    measure_spread_factor(solver.phi, solver.solid) gives the device
+    measure()'s extents, and Monitor.record() fills its history."""
+    need_gpu()
+    print("[gpu] reference measure helpers", flush=True)
+    from io_utils import Monitor
+    from lbm.fe_ac_solver import AllenCahnSolver
+    from lbm.fe_config import FEConfig
+    from lbm.fe_measure import measure_contact_angle, measure_spread_factor
+    z, m = load("ridge162_readme_f32")
+    cfg = FEConfig(**config_kwargs(m), device="cuda")
+    s = AllenCahnSolver(cfg, dtype=torch.float32, **solver_kwargs(m))
+    s.set_solid(z["solid"], z["fraction"])
+    s.init_fields(z["phi_init"], z["u_init"])
+    mon = Monitor(cfg)
+    for _ in range(3):
+        s.step(5)
+        mon.record(s)
+        host = measure_spread_factor(s.phi, s.solid)
+        dev = s.measure()
+        for k in ("Dx", "Dy", "Dz", "k", "C_max"):
+            assert host[k] == dev[k], k
+        theta = measure_contact_angle(s.phi, s.solid)
+        assert 0.0 <= theta <= 180.0
+    assert mon.history["step"] == [5, 10, 15]
+    assert mon.history["mass"][-1] == float(s.rho.cpu().numpy().sum())
