@@ -37,9 +37,20 @@ def inject_event(m):
                           tuple(inj["u"]))
 
 
-def solver_kwargs(m):
+def solver_kwargs(m, z=None, device=None):
+    """Solver keyword arguments of a case; a per-node amplification field
+    ("field" in the metadata) comes from the fixture, as a numpy array for
+    the oracle or a torch tensor on `device` for the solver."""
     kw = dict(m["kw"])
     kw.setdefault("boundary_relax", 0.3)
+    if kw.get("geo_amplification") == "field":
+        if z is None:
+            raise ValueError("this case needs its fixture for the amplification field")
+        amp = np.asarray(z["amp_field"])
+        if device is not None:
+            import torch
+            amp = torch.from_numpy(amp).to(device)
+        kw["geo_amplification"] = amp
     return kw
 
 

@@ -153,6 +153,19 @@ CASES = {
                 interface_mode="cahn-hilliard"),
         dtype="float32", center=(10.0, 10.0, 10.0), radius=4.0,
         u=(0.0, 0.0, -0.05), frac=True, tau_h=None, fg_steps=(1,)),
+    # per-node amplification field (tensor geo_amplification,
+    # lbm/fe_wetting.py:503-512): values in [1, 2], exactly 1 (un-amplified)
+    # on some nodes
+    "ridge144_alphafield_2d_f64": dict(
+        grid=(32, 32, 1), sub="ridge", R_star=1.0, R_d=5.0, theta=144.0,
+        kw=dict(geo_amplification="field", geometric_wetting=True),
+        dtype="float64", center=(16.0, 11.5), radius=5.0,
+        u=(0.0, -0.04), frac=True, tau_h=None, fg_steps=(1,)),
+    "ridge120_alphafield_f32": dict(
+        grid=(20, 20, 16), sub="ridge", R_star=1.0, R_d=5.0, theta=120.0,
+        kw=dict(geo_amplification="field", geometric_wetting=True),
+        dtype="float32", center=(10.0, 10.0, 9.0), radius=4.0,
+        u=(0.0, 0.0, -0.05), frac=True, tau_h=None, fg_steps=()),
     # a second drop injected after step 3 (3D, fp64) and after step 2 (2D, fp32)
     "ridge162_inject_f64": dict(
         grid=(20, 20, 16), sub="ridge", R_star=1.0, R_d=5.0, theta=162.0,
@@ -173,6 +186,16 @@ CASES = {
 SAVE_STEPS = (1, 2, 5, 20, 60)
 
 
+def amp_field(grid):
+    """A per-node alpha in [1, 2] (float32, as scripts/adaptive_alpha_study.py
+    builds it), exactly 1 where the sine vanishes."""
+    nx, ny, nz = grid
+    shape = (nx, ny) if nz == 1 else (nx, ny, nz)
+    idx = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    a = 1.0 + 0.9 * np.abs(np.sin(0.35 * idx[0]) * np.cos(0.21 * idx[1]))
+    return np.clip(a, 1.0, 2.0).astype(np.float32)
+
+
 def run_case(name, c):
     nx, ny, nz = c["grid"]
     dt = torch.float32 if c["dtype"] == "float32" else torch.float64
@@ -185,8 +208,12 @@ def run_case(name, c):
         cfg_kw["g_force"] = tuple(c["g_force"]) + (0.0,) * (3 - len(c["g_force"]))
     cfg = FEConfig(**cfg_kw)
     out = {}
+    kw = dict(c["kw"])
+    if kw.get("geo_amplification") == "field":
+        out["amp_field"] = amp_field(c["grid"])
+        kw["geo_amplification"] = torch.from_numpy(out["amp_field"])
     with contextlib.redirect_stdout(io.StringIO()):
-        sol = AllenCahnSolver(cfg, dtype=dt, **c["kw"])
+        sol = AllenCahnSolver(cfg, dtype=dt, **kw)
         if c["frac"]:
             solid, frac = create_substrate_with_fraction(
                 nx, ny, nz, substrate_type=c["sub"], R_star=c["R_star"],

@@ -34,7 +34,7 @@ def gpu_solver(name, dtype):
     from lbm.fe_config import FEConfig
     z, m = load(name)
     cfg = FEConfig(**config_kwargs(m), device="cuda")
-    s = AllenCahnSolver(cfg, dtype=dtype, **solver_kwargs(m))
+    s = AllenCahnSolver(cfg, dtype=dtype, **solver_kwargs(m, z, "cuda"))
     s.set_solid(z["solid"], z["fraction"] if "fraction" in z.files else None)
     s.init_fields(z["phi_init"], z["u_init"])
     return s, z, m
@@ -42,7 +42,7 @@ def gpu_solver(name, dtype):
 
 def oracle(name, dtype):
     z, m = load(name)
-    o = OracleSolver(Params(**config_kwargs(m)), dtype=dtype, **solver_kwargs(m))
+    o = OracleSolver(Params(**config_kwargs(m)), dtype=dtype, **solver_kwargs(m, z))
     o.set_solid(z["solid"], z["fraction"] if "fraction" in z.files else None)
     o.init_fields(z["phi_init"], z["u_init"])
     return o

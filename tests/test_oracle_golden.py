@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-from _cases import inject_event
+from _cases import inject_event, solver_kwargs
 from oracle.ac_oracle import OracleSolver, Params
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -36,9 +36,8 @@ def build(name):
         kw["g_force"] = tuple(m["g_force"]) + (0.0,) * (3 - len(m["g_force"]))
     p = Params(**kw)
     dt = np.float32 if m["dtype"] == "float32" else np.float64
-    skw = dict(m["kw"])
-    skw.setdefault("boundary_relax", 0.3)
-    o = OracleSolver(p, dtype=dt, **{k: v for k, v in skw.items()})
+    skw = solver_kwargs(m, z)
+    o = OracleSolver(p, dtype=dt, **skw)
     frac = z["fraction"] if "fraction" in z.files else None
     o.set_solid(z["solid"], frac)
     o.init_fields(z["phi_init"], z["u_init"])
@@ -104,9 +103,7 @@ def _twin64(name):
     """The same case in float64 (the rounding-free reference trajectory)."""
     z, m = load(name)
     o, _, _ = build(name)
-    skw = dict(m["kw"])
-    skw.setdefault("boundary_relax", 0.3)
-    t = OracleSolver(o.p, dtype=np.float64, **skw)
+    t = OracleSolver(o.p, dtype=np.float64, **solver_kwargs(m, z))
     t.set_solid(z["solid"], z["fraction"] if "fraction" in z.files else None)
     t.init_fields(z["phi_init"], z["u_init"])
     return t
