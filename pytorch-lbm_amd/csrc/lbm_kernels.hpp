@@ -828,7 +828,12 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   const Buf bFi = dist_buf<DIM>(a.Fin, N), bGi = dist_buf<DIM>(a.Gin, N);
   const Buf bPp = field_buf(a.phi_prev, N), bPc = field_buf(a.phi_cur, N);
   const Buf bU = field_buf(a.u, N, DIM);
+  // 16-bit storage (VALU-bound): a wave issuing its pulls runs at raised
+  // priority, so its loads leave early while other waves compute (measured
+  // 142.1 -> 140.6 us/step for bf16; no gain for 4-byte storage)
+  if constexpr (DEV) __builtin_amdgcn_s_setprio(1);
   pull<T, S, DIM, false, false, LOCAL, CP_G>(bGi, bGi, x, nb, m, a.eps, bPp, ph, gg, unused);
+  if constexpr (DEV) __builtin_amdgcn_s_setprio(0);
   T pprev = T(0);
   if (((m & M_TOP) && !LOCAL) || a.phiu_valid) pprev = a.phi_prev[x];
 
@@ -1008,7 +1013,9 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   asm volatile("" : "+v"(x2));
   const Nbrs<DIM> nb2 = neighbours<DIM>(g, x2, decompose<DIM>(g, x2));
   T f[Q];
+  if constexpr (DEV) __builtin_amdgcn_s_setprio(1);
   pull<T, S, DIM, DEV, false, LOCAL, CP_FLD>(bFi, bFi, x, nb2, m, a.eps, bPp, ph, f, unused);
+  if constexpr (DEV) __builtin_amdgcn_s_setprio(0);
   if constexpr (!LOCAL)
     scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot_prev], m, pprev, nullptr,
                     nsw_phi<T>(ph, m, a.phi_prev, x));
