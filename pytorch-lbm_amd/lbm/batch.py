@@ -218,6 +218,13 @@ class BatchEngine:
         keys = ("Dx", "Dy", "Dz", "k", "volume", "C_max", "z_com", "n_liquid", "phi_sum", "nan")
         return {k: float(v) for k, v in zip(keys, out)}
 
+    def measure_async(self, cid, out):
+        """measure() of case cid written to `out` (10 float64 on the device)
+        without synchronising."""
+        nat.check(nat.lib.lbm_measure_async(self._solver(cid), nat.ptr(out), self._stream()),
+                  "lbm_measure_async")
+        return out
+
     def contact_radius(self, cid):
         spec = self.cases[cid]
         out = ctypes.c_double()

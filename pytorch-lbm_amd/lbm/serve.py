@@ -20,6 +20,13 @@ request, and optional checkpoints hold the full device state of every
 resident case; a restarted server skips finished labels and resumes
 unfinished ones from their last checkpoint, with results identical to an
 uninterrupted run.
+
+Failure isolation (the reference's drivers stop a case whose phi holds a NaN
+or has vanished, max(phi) < 0.01, at every 50-step measurement,
+scripts/run_thesis_sweep.py:214-216): after every chunk each resident case's
+diagnostics are reduced on the device and read back in one transfer; a failed
+case is measured, recorded with stable = False and evicted, and the cases
+sharing its launches are unaffected.
 """
 from __future__ import annotations
 
@@ -126,7 +133,8 @@ class SweepServer:
 
     def __init__(self, storage="bf16", budget_bytes=0, max_active=32, chunk=50,
                  device=None, store: Optional[ResultStore] = None,
-                 checkpoint_dir: Optional[str] = None, checkpoint_every=0):
+                 checkpoint_dir: Optional[str] = None, checkpoint_every=0,
+                 health_check=True):
         self.engine = BatchEngine(storage, budget_bytes=budget_bytes, device=device)
         self.storage = storage
         self.budget = int(budget_bytes)
@@ -138,6 +146,7 @@ class SweepServer:
         self.store = store
         self.ckpt_dir = checkpoint_dir
         self.ckpt_every = int(checkpoint_every)
+        self.health_check = bool(health_check)
         if checkpoint_dir:
             os.makedirs(checkpoint_dir, exist_ok=True)
 
@@ -162,6 +171,20 @@ class SweepServer:
         self.engine.load_case(req.case_id, state)
         return True
 
+    def _unstable(self):
+        """Resident requests whose phi holds a NaN or has vanished: every
+        case's device diagnostics, one read-back."""
+        import torch
+        ids = list(self.active)
+        if not ids:
+            return []
+        rec = torch.empty((len(ids), 10), dtype=torch.float64, device=self.engine.device)
+        for j, cid in enumerate(ids):
+            self.engine.measure_async(cid, rec[j])
+        rows = rec.cpu().numpy()            # columns 5: max(phi), 9: NaN flag
+        return [self.active[cid] for j, cid in enumerate(ids)
+                if rows[j, 9] != 0 or not rows[j, 5] >= 0.01]
+
     def _fits(self, req):
         if len(self.active) >= self.max_active:
             return False
@@ -169,12 +192,13 @@ class SweepServer:
             return True
         return self.engine.bytes_used() + footprint(req.spec, self.storage) <= self.budget
 
-    def _finish(self, req, now):
+    def _finish(self, req, now, stable=True):
         cid = req.case_id
         m = self.engine.measure(cid)
         req.result = {"Dx": m["Dx"], "Dy": m["Dy"], "Dz": m["Dz"], "z_com": m["z_com"],
                       "volume": m["volume"], "nan": m["nan"],
-                      "contact_radius": self.engine.contact_radius(cid)}
+                      "contact_radius": self.engine.contact_radius(cid),
+                      "stable": stable, "steps_done": self.engine.status(cid)[0]}
         if self.keep_fields:
             req.result["phi"] = self.engine.field(cid, "phi").cpu()
         req.t_done = now
@@ -238,6 +262,11 @@ class SweepServer:
             self.lattice_updates += n * sum(r.spec.nodes for r in self.active.values())
             torch.cuda.current_stream(self.engine.device).synchronize()
             now = time.perf_counter() - t0
+            if self.health_check:
+                for req in self._unstable():
+                    self._finish(req, now, stable=False)
+                    if on_event:
+                        on_event("failed", req)
             for cid, req in list(self.active.items()):
                 k, target = self.engine.status(cid)
                 if k >= target:

@@ -107,3 +107,36 @@ def test_preempted_server_resumes_identically(tmp_path):
                                       ref[req.spec.tag].result["phi"].numpy())
     import os
     assert not [f for f in os.listdir(ckpt) if f.endswith(".npy")]
+
+
+def test_failed_case_is_isolated():
+    """A case whose state turns to NaN (forced from the host: a block of its
+    distributions overwritten through a state image at admission) is stopped
+    after its first chunk and recorded unstable; every other request still
+    gets the result of the undisturbed run, bit for bit."""
+    need_gpu()
+    print("[gpu] serve failure isolation", flush=True)
+    from lbm.serve import SweepServer
+    ref = {r.spec.tag: r for r in SweepServer("bf16", max_active=3, chunk=8).run(
+        small_stream(), keep_fields=True)}
+    srv = SweepServer("bf16", max_active=3, chunk=8)
+    events = []
+
+    def on_event(kind, req):
+        events.append((kind, req.spec.tag))
+        if kind == "admit" and req.spec.tag == "t1":
+            img = srv.engine.save_case(req.case_id)
+            img[256:256 + 4096] = 0xFF           # NaN in the first distribution block
+            srv.engine.load_case(req.case_id, img)
+    done = {r.spec.tag: r for r in srv.run(small_stream(), keep_fields=True, on_event=on_event)}
+    assert ("failed", "t1") in events
+    bad = done["t1"]
+    assert bad.result["stable"] is False and bad.result["nan"] == 1.0
+    assert bad.result["steps_done"] < bad.spec.steps
+    for tag, req in done.items():
+        if tag == "t1":
+            continue
+        assert req.result["stable"] is True
+        np.testing.assert_array_equal(req.result["phi"].numpy(), ref[tag].result["phi"].numpy())
+        for k in ("Dx", "Dy", "Dz", "volume", "contact_radius"):
+            assert req.result[k] == ref[tag].result[k]
