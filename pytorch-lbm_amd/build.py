@@ -26,6 +26,11 @@ FLAGS = [
 ]
 
 
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+DRIVER_SRC = os.path.join(os.path.dirname(HERE), "tests", "c_abi", "abi_driver.cpp")
+DRIVER = os.path.join(os.path.dirname(DRIVER_SRC), "abi_driver")
+
+
 def build(verbose=True):
     os.makedirs(OUT_DIR, exist_ok=True)
     cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp", SRC]
@@ -33,7 +38,25 @@ def build(verbose=True):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(OUT + ".tmp", OUT)
+    build_driver(verbose)
     return OUT
+
+
+def build_driver(verbose=True):
+    """Host-only C++ caller of the C ABI (tests/c_abi): g++, linked against
+    the library above and the HIP runtime, no torch."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    rel = os.path.relpath(OUT_DIR, os.path.dirname(DRIVER))
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__",
+           "-I" + os.path.join(ROCM, "include"), DRIVER_SRC, "-o", DRIVER + ".tmp",
+           "-L" + OUT_DIR, "-llbm_amd", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":
