@@ -1125,7 +1125,8 @@ int lbm_inject_droplet(lbm_solver* s, const void* c_dev, const double* u_impact,
 // ---------------------------------------------------------------------------
 // Solver state snapshot (preemption / checkpoint).  Layout of the byte image:
 //   [0, 256)   StateHeader
-//   then       F[cur], G[cur], G[cur^1] (AoSoA, the step's distributions and
+//   then       F[cur], G[cur], G[cur^1] (AoSoA with the header's block size,
+//              the step's distributions and
 //              the previous g set the P observable is pulled from), u,
 //              phi[0], phi[1], Ctrl, kept P, kept mu
 // each array starting at a 256-byte boundary.  The kept P and mu (observables
@@ -1137,11 +1138,11 @@ struct StateHeader {
   uint32_t magic, version;
   int32_t nx, ny, nz, precision;
   long long k;
-  int32_t cur, local, prev_local, phiu_valid, kept, pad;
+  int32_t cur, local, prev_local, phiu_valid, kept, block;   // block: AoSoA nodes per block
   uint64_t bytes;
 };
 constexpr uint32_t kStateMagic = 0x4c424d53u;   // "LBMS"
-constexpr uint32_t kStateVersion = 2;
+constexpr uint32_t kStateVersion = 3;
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
@@ -1179,6 +1180,7 @@ int lbm_save_state(lbm_solver* s, void* dst, void* stream) {
   h.k = s->k;
   h.cur = s->cur; h.local = s->local; h.prev_local = s->prev_local; h.phiu_valid = s->phiu_valid;
   h.kept = s->kept;
+  h.block = kAB;
   h.bytes = L.total;
   char* d = (char*)dst;
   const void* src[9] = {s->Fb[s->cur], s->Gb[s->cur], s->Gb[s->cur ^ 1], s->u, s->phi[0],
@@ -1200,8 +1202,9 @@ int lbm_load_state(lbm_solver* s, const void* src, void* stream) {
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipMemcpy(&h, src, sizeof(h), hipMemcpyDefault));
   const StateLayout L = state_layout(s);
-  if (h.magic != kStateMagic || h.version != kStateVersion)
-    return fail(LBM_EINVAL, "not a solver state image");
+  if (h.magic != kStateMagic) return fail(LBM_EINVAL, "not a solver state image");
+  if (h.version != kStateVersion || h.block != kAB)
+    return fail(LBM_EINVAL, "state image written with another distribution layout");
   if (h.nx != s->p.nx || h.ny != s->p.ny || h.nz != s->p.nz || h.precision != s->p.precision ||
       h.bytes != L.total)
     return fail(LBM_EINVAL, "state image is for another grid or precision");

@@ -140,35 +140,43 @@ __device__ __forceinline__ void bput(const Buf& b, uint32_t N, int i, uint32_t i
                             Stor<S>::template from<T>(v));
 }
 
-// Distribution layout (AoSoA): nodes are grouped in blocks of 64 (one wave of
-// consecutive nodes); a block stores its Q directions one after another, 64
-// values each, so element (i, x) sits at ((x / 64) * Q + i) * 64 + x % 64.  A
-// wave's node-local stores of all directions fill one contiguous region
-// instead of Q regions N elements apart, and pulls read 64-value runs of one
-// direction as before.  Buffers hold padded_nodes(N) nodes.
-__host__ __device__ constexpr long long padded_nodes(long long N) { return (N + 63) & ~63LL; }
+// Distribution layout (AoSoA): nodes are grouped in blocks of kAB consecutive
+// nodes; a block stores its Q directions one after another, kAB values each,
+// so element (i, x) sits at ((x / kAB) * Q + i) * kAB + x % kAB.  A wave's
+// node-local stores of all directions fill one contiguous region instead of
+// Q regions N elements apart, and pulls read 64-value runs of one direction.
+// Buffers hold padded_nodes(N) nodes.
+#ifndef LBM_AOSOA_BLOCK
+#define LBM_AOSOA_BLOCK 256
+#endif
+constexpr int kAB = LBM_AOSOA_BLOCK;
+static_assert(kAB >= 64 && (kAB & (kAB - 1)) == 0, "AoSoA block: a power of two >= 64");
+constexpr int kABsh = __builtin_ctz(kAB);
+__host__ __device__ constexpr long long padded_nodes(long long N) {
+  return (N + (kAB - 1)) & ~(long long)(kAB - 1);
+}
 
 template <int Q, typename S> __device__ __forceinline__ uint32_t dvoff(uint32_t idx) {
-  // = ((idx / 64) * Q * 64 + idx % 64) * sizeof(S), with a full-rate 24-bit
-  // multiply (idx / 64 < 2^24; the product stays below 2^32 because a
+  // = ((idx / kAB) * Q * kAB + idx % kAB) * sizeof(S), with a full-rate 24-bit
+  // multiply (idx / kAB < 2^24; the product stays below 2^32 because a
   // distribution set is smaller than 4 GiB)
-  return __umul24(idx >> 6, (uint32_t)((Q - 1) * 64 * sizeof(S))) + idx * (uint32_t)sizeof(S);
+  return __umul24(idx >> kABsh, (uint32_t)((Q - 1) * kAB * sizeof(S))) + idx * (uint32_t)sizeof(S);
 }
 // load of direction i at the node whose dvoff is vo
 template <int Q, typename T, typename S, int CP = 0>
 __device__ __forceinline__ T dload(const Buf& b, int i, uint32_t vo) {
   return Stor<S>::template to<T>(
-      BufIO<S>::template ld<CP>(b, vo, (uint32_t)i * 64u * (uint32_t)sizeof(S)));
+      BufIO<S>::template ld<CP>(b, vo, (uint32_t)i * (uint32_t)kAB * (uint32_t)sizeof(S)));
 }
 template <int Q, typename T, typename S>
 __device__ __forceinline__ T dget(const Buf& b, int i, uint32_t idx) {
   return Stor<S>::template to<T>(
-      BufIO<S>::ld(b, dvoff<Q, S>(idx), (uint32_t)i * 64u * (uint32_t)sizeof(S)));
+      BufIO<S>::ld(b, dvoff<Q, S>(idx), (uint32_t)i * (uint32_t)kAB * (uint32_t)sizeof(S)));
 }
 // store of direction i at the node whose dvoff is vo
 template <int Q, typename T, typename S, int CP = 0>
 __device__ __forceinline__ void dput(const Buf& b, int i, uint32_t vo, T v) {
-  BufIO<S>::template st<CP>(b, vo, (uint32_t)i * 64u * (uint32_t)sizeof(S),
+  BufIO<S>::template st<CP>(b, vo, (uint32_t)i * (uint32_t)kAB * (uint32_t)sizeof(S),
                             Stor<S>::template from<T>(v));
 }
 

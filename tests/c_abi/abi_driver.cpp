@@ -172,6 +172,20 @@ int main() {
   OK(lbm_step(a.s, 5, nullptr));
   std::vector<float> p25b = read_phi(a.s);
   CHECK(std::memcmp(p25.data(), p25b.data(), N * 4) == 0, "resumed run differs");
+  {
+    // a damaged magic word or an older format version is refused
+    std::vector<unsigned char> bad(img);
+    bad[0] ^= 0xff;
+    CHECK(lbm_load_state(a.s, bad.data(), nullptr) == LBM_EINVAL, "bad magic accepted");
+    bad = img;
+    uint32_t v = 0;
+    std::memcpy(&v, bad.data() + 4, 4);
+    v -= 1;
+    std::memcpy(bad.data() + 4, &v, 4);
+    CHECK(lbm_load_state(a.s, bad.data(), nullptr) == LBM_EINVAL, "old version accepted");
+    CHECK(std::strstr(lbm_last_error(), "layout") != nullptr, "message: %s", lbm_last_error());
+    CHECK(lbm_step_count(a.s) == 25, "refused image changed the solver");
+  }
 
   // a second drop between steps: the mass target grows
   double m_before = 0.0;
