@@ -338,6 +338,16 @@ def serve_bench(duration=20.0, rate=4.0, seed=5, storage="bf16", slots=32, chunk
     }
 
 
+def graph_chunk(steps, most=50):
+    """Largest even g <= most dividing steps (steps - 1 when steps is odd);
+    0 when fewer than two steps are timed."""
+    n = steps - (steps & 1)
+    for g in range(min(most, n) & ~1, 1, -2):
+        if n % g == 0:
+            return g
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -345,6 +355,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--storage", default=None, choices=[None, "fp32", "bf16", "fp16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager launches instead of replays of a captured step graph")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape (256-thread blocks, no occupancy cap)")
     ap.add_argument("--cpu-steps", type=int, default=6,
@@ -389,12 +401,24 @@ def main():
     b_coll = B_COLLIDE_16 if sixteen else B_COLLIDE
     if not args.no_autotune:
         s.autotune(5)               # part of the warmup: 108 real steps
-    s.step(args.warmup)
+    s.step(max(args.warmup, 1))
+    # the timed steps run as replays of a captured graph of g steps (even,
+    # dividing the step count; an odd count runs its last step eagerly),
+    # replayed once untimed first
+    g = 0 if args.eager else graph_chunk(args.steps)
+    if g:
+        s.capture_graph(g)
+        s.run_graph(1)
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     start.record()
-    s.step(args.steps)
+    if g:
+        s.run_graph(args.steps // g)
+        if args.steps % g:
+            s.step(args.steps % g)
+    else:
+        s.step(args.steps)
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -436,6 +460,7 @@ def main():
                      "bytes_per_lu": b_coll, "avg_ms": round(tb, 5),
                      "launch": dict(zip(("block_phi", "block_collide", "cap_phi",
                                          "cap_collide", "split"), s.launch()))},
+        "timing": (f"replays of a captured {g}-step graph" if g else "eager launches"),
         "kernels_ms": {"k_phi": round(ta, 5), "k_collide": round(tb, 5),
                        "step_events": round(tstep, 5)},
         "mass_drift": abs(mass - mass0) / mass0,
