@@ -218,7 +218,9 @@ int lbm_inject_droplet(lbm_solver* s, const void* c_dev, const double* u_impact,
  * both phi buffers, mass factors, step count -- but not the geometry: restore
  * into a solver of the same grid and precision after the same
  * lbm_set_geometry.  `dst` / `src` may be device or host memory of
- * lbm_state_bytes() bytes.  Both synchronise `stream`. */
+ * lbm_state_bytes() bytes.  Both synchronise `stream`.  lbm_load_state
+ * returns LBM_EINVAL, leaving the solver unchanged, for an image of another
+ * grid, precision, format version or distribution layout. */
 size_t lbm_state_bytes(const lbm_solver* s);
 int lbm_save_state(lbm_solver* s, void* dst, void* stream);
 int lbm_load_state(lbm_solver* s, const void* src, void* stream);
