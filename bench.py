@@ -400,7 +400,7 @@ def main():
     b_step = B_STEP_16 if sixteen else B_STEP
     b_coll = B_COLLIDE_16 if sixteen else B_COLLIDE
     if not args.no_autotune:
-        s.autotune(5)               # part of the warmup: 108 real steps
+        s.autotune(5)               # part of the warmup: 162 real steps
     s.step(max(args.warmup, 1))
     # the timed steps run as replays of a captured graph of g steps (even,
     # dividing the step count; an odd count runs its last step eagerly),

@@ -157,19 +157,23 @@ int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide);
  * No counterpart in the reference. */
 int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide);
 
-/* Pass B as two launches over 64-node chunk lists (1, default): plain-fluid
- * chunks with a wall-free kernel (fewer registers, more resident waves), the
- * other chunks with the general kernel; or one launch over all nodes (0).
- * Results are identical either way.  No counterpart in the reference. */
+/* Form of pass B.  1 (default): two launches over 64-node chunk lists,
+ * plain-fluid chunks with a wall-free kernel (fewer registers, more resident
+ * waves), the other chunks with the general kernel.  2: one launch over both
+ * lists, arranged so that each XCD runs its plain chunks first and its wall
+ * chunks last (the wall chunks fill the launch's tail; the kernel carries the
+ * general code's registers).  0: one launch over all nodes.  Results are
+ * identical in every form; the autotuner picks one.  LBM_EINVAL for other
+ * values.  No counterpart in the reference. */
 int lbm_set_split(lbm_solver* s, int split);
 
 /* Current launch shape: out5 = {block size A, block size B, cap A, cap B,
- * split}. */
+ * form of pass B (lbm_set_split)}. */
 int lbm_get_launch(const lbm_solver* s, int* out5);
 
-/* Autotuner: times candidate (block size, occupancy cap[, split]) shapes for
+/* Autotuner: times candidate (block size, occupancy cap[, form]) shapes for
  * pass A and pass B over `nsteps_each` real steps each (the solver advances
- * by 18*(nsteps_each+1) steps) and keeps the fastest of each; reports the
+ * by 27*(nsteps_each+1) steps) and keeps the fastest of each; reports the
  * block sizes in *best_phi / *best_collide (lbm_get_launch gives the rest).
  * Synchronises `stream`.  No counterpart in the reference. */
 int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,

@@ -10,6 +10,7 @@
 // synchronises, so step sequences are graph-capturable.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -39,7 +40,7 @@ static int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int BS = 256;
-constexpr int kTuneRounds = 18;  // autotuner rounds (steps: rounds * (n + 1))
+constexpr int kTuneRounds = 27;  // autotuner rounds (steps: rounds * (n + 1))
 
 struct lbm_solver {
   lbm_params p;
@@ -88,7 +89,13 @@ struct lbm_solver {
   uint32_t* chunks = nullptr;
   uint8_t* chunk_flag = nullptr;
   unsigned int n_plain = 0, n_wall = 0;
-  bool split = true;          // run pass B as the two chunk lists
+  // the same chunks as one list per pass-B block size (128, 256, 512) for the
+  // single-launch form: per XCD range, plain chunks then wall chunks
+  uint32_t* mixed[3] = {nullptr, nullptr, nullptr};
+  unsigned int n_mixed[3] = {0, 0, 0};
+  // pass B: 0 one launch over all nodes, 1 the two chunk-list launches,
+  // 2 one launch over the per-XCD mixed list
+  int split = 1;
   int capA = 0, capB = 0;     // occupancy caps, waves per SIMD (0: none)
   unsigned int ldsA = 0, ldsB = 0;   // dynamic LDS per block enforcing the caps
   hipGraphExec_t gexec = nullptr;
@@ -377,11 +384,41 @@ int lbm_destroy(lbm_solver* s) {
   void* ptrs[] = {s->Fb[0], s->Fb[1], s->Gb[0], s->Gb[1], s->u, s->phi[0], s->phi[1],
                   s->eps, s->ampf, s->meta, s->bnd, s->dist[0], s->dist[1], s->solid,
                   s->wmask, s->normal, s->surf, s->partials, s->mparts, s->dout, s->ctrl,
-                  s->chunks, s->chunk_flag, s->keep, s->nsw, s->mu};
+                  s->chunks, s->chunk_flag, s->keep, s->nsw, s->mu,
+                  s->mixed[0], s->mixed[1], s->mixed[2]};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete s;
   return LBM_OK;
+}
+
+// The single-launch pass-B list for blocks of `wpb` waves: the launch has
+// G = ceil(n / wpb) blocks, and xcd_block gives XCD j the logical blocks
+// [j*(G/8) + min(j, G%8), ...) of G/8 (+1 for j < G%8).  Each XCD's range
+// gets a contiguous slice of the plain chunks, then a slice of the wall
+// chunks (bit 31 set) in proportion to its capacity; padding entries are
+// 0xFFFFFFFF.  `list` holds the plain chunks, then the wall chunks.
+static std::vector<uint32_t> mixed_list(const std::vector<uint32_t>& list, unsigned int np,
+                                        unsigned int nw, unsigned int wpb) {
+  const unsigned int n = np + nw, G = (n + wpb - 1) / wpb;
+  const unsigned int per = G / kXCD, rem = G % kXCD;
+  std::vector<uint32_t> mx((size_t)G * wpb, 0xFFFFFFFFu);
+  size_t pp = 0, ww = 0, left = mx.size();
+  for (unsigned int j = 0; j < (unsigned int)kXCD; ++j) {
+    const size_t start = ((size_t)j * per + (j < rem ? j : rem)) * wpb;
+    const size_t cap = ((size_t)per + (j < rem ? 1 : 0)) * wpb;
+    size_t wj = left ? (size_t)((double)(nw - ww) * (double)cap / (double)left + 0.5) : 0;
+    wj = std::min(std::min(wj, (size_t)nw - ww), cap);
+    size_t pj = std::min(cap - wj, (size_t)np - pp);
+    wj = std::min(cap - pj, (size_t)nw - ww);   // fill what the plain share leaves
+    for (size_t t = 0; t < pj; ++t) mx[start + t] = list[pp + t];
+    for (size_t t = 0; t < wj; ++t) mx[start + pj + t] = list[np + ww + t] | 0x80000000u;
+    pp += pj;
+    ww += wj;
+    left -= cap;
+  }
+  // capacity G * wpb >= n and every range is filled before the next: all placed
+  return mx;
 }
 
 template <typename T, int DIM>
@@ -433,6 +470,14 @@ static int set_geometry_impl(lbm_solver* s, const void* frac, int fdt, hipStream
   s->n_wall = (unsigned int)(nch - s->n_plain);
   HIPCHK(hipMemcpyAsync(s->chunks, list.data(), nch * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
+  for (int k = 0; k < 3; ++k) {
+    const std::vector<uint32_t> mx = mixed_list(list, s->n_plain, s->n_wall, 2u << k);
+    if (s->mixed[k]) HIPCHK(hipFree(s->mixed[k]));
+    s->mixed[k] = nullptr;
+    HIPCHK(hipMalloc(&s->mixed[k], mx.size() * 4));
+    HIPCHK(hipMemcpy(s->mixed[k], mx.data(), mx.size() * 4, hipMemcpyHostToDevice));
+    s->n_mixed[k] = (unsigned int)mx.size();
+  }
   return LBM_OK;
 }
 
@@ -642,8 +687,9 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
       LBM_BS_SWITCH(s->bsB, LAUNCH_B)
 #undef LAUNCH_B
     } else {
-      // plain-fluid chunks, then the rest; the first launch forms the mass
-      // factor
+      // plain-fluid chunks, then the rest, as two launches (the first forms
+      // the mass factor) or as one launch over the mixed list (the first step
+      // after init_fields, in the local form, always takes the two launches)
       const uint32_t* cp = s->chunks;
       const uint32_t* cw = s->chunks + s->n_plain;
       const unsigned int np = s->n_plain, nw = s->n_wall;
@@ -653,6 +699,10 @@ static int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev =
     if (s->local) {                                                                         \
       if (np) k_collide_chunks<T, S, DIM, B, true, true><<<gp, B, s->ldsB, st>>>(a, ph, cp, np, 1); \
       if (nw) k_collide_chunks<T, S, DIM, B, true, false><<<gw, B, s->ldsB, st>>>(a, ph, cw, nw, np == 0); \
+    } else if (s->split == 2) {                                                             \
+      const int k = B == 128 ? 0 : (B == 256 ? 1 : 2);                                      \
+      k_collide_mixed<T, S, DIM, B, false><<<s->n_mixed[k] / (B / 64), B, s->ldsB, st>>>(   \
+          a, ph, s->mixed[k], s->n_mixed[k]);                                                \
     } else {                                                                                \
       if (np) k_collide_chunks<T, S, DIM, B, false, true><<<gp, B, s->ldsB, st>>>(a, ph, cp, np, 1); \
       if (nw) k_collide_chunks<T, S, DIM, B, false, false><<<gw, B, s->ldsB, st>>>(a, ph, cw, nw, np == 0); \
@@ -827,14 +877,15 @@ int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide) {
 
 int lbm_set_split(lbm_solver* s, int split) {
   if (!s) return fail(LBM_EINVAL, "null solver");
-  s->split = split != 0;
+  if (split < 0 || split > 2) return fail(LBM_EINVAL, "split must be 0, 1 or 2");
+  s->split = split;
   return lbm_set_block_sizes(s, s->bsA, s->bsB);
 }
 
 int lbm_get_launch(const lbm_solver* s, int* out5) {
   if (!s || !out5) return fail(LBM_EINVAL, "null argument");
   out5[0] = s->bsA; out5[1] = s->bsB; out5[2] = s->capA; out5[3] = s->capB;
-  out5[4] = s->split ? 1 : 0;
+  out5[4] = s->split;
   return LBM_OK;
 }
 
@@ -843,13 +894,18 @@ int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,
   if (!s) return fail(LBM_EINVAL, "null solver");
   if (!s->inited) return fail(LBM_ESTATE, "autotune before init_fields()");
   if (nsteps_each < 2) nsteps_each = 2;
-  // (block size, occupancy cap, chunk split) candidates; pass A and pass B
+  // (block size, occupancy cap, pass-B form) candidates; pass A and pass B
   // are timed independently (events around each pass), so round k times
   // candidate k of pass B together with candidate k % 6 of pass A
-  const int candB[kTuneRounds][3] = {
-      {128, 0, 0}, {256, 0, 0}, {512, 0, 0}, {128, 5, 0}, {256, 5, 0}, {512, 5, 0},
-      {128, 4, 0}, {256, 4, 0}, {512, 4, 0}, {128, 0, 1}, {256, 0, 1}, {512, 0, 1},
-      {128, 5, 1}, {256, 5, 1}, {512, 5, 1}, {128, 4, 1}, {256, 4, 1}, {512, 4, 1}};
+  int candB[kTuneRounds][3];
+  {
+    const int bsz[3] = {128, 256, 512}, caps[3] = {0, 5, 4};
+    for (int k = 0; k < kTuneRounds; ++k) {
+      candB[k][0] = bsz[k % 3];
+      candB[k][1] = caps[(k / 3) % 3];
+      candB[k][2] = k / 9;
+    }
+  }
   const int candA[6][2] = {{128, 0}, {256, 0}, {512, 0}, {128, 6}, {256, 6}, {512, 6}};
   double bestA = 1e30, bestB = 1e30;
   int bA[2] = {256, 0}, bB[3] = {256, 0, 1};
@@ -858,7 +914,7 @@ int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,
     const int* b = candB[c];
     s->capA = a[1];
     s->capB = b[1];
-    s->split = b[2] != 0;
+    s->split = b[2];
     int rc = lbm_set_block_sizes(s, a[0], b[0]);
     if (rc) return rc;
     double t[3];
@@ -871,7 +927,7 @@ int lbm_autotune(lbm_solver* s, int nsteps_each, void* stream, int* best_phi,
   }
   s->capA = bA[1];
   s->capB = bB[1];
-  s->split = bB[2] != 0;
+  s->split = bB[2];
   int rc = lbm_set_block_sizes(s, bA[0], bB[0]);
   if (rc) return rc;
   if (best_phi) *best_phi = bA[0];

@@ -1079,6 +1079,28 @@ __global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_collide_chunks(
   collide_node<T, S, DIM, LOCAL, FLUID>(a, ph, x);
 }
 
+// Pass B in one launch over a chunk list arranged on the host so that each
+// XCD's range of logical blocks (xcd_block) holds its share of the plain
+// chunks first and its share of the wall chunks (bit 31 set) last: the wall
+// chunks, whose waves live as long as plain ones but run the general code,
+// fill the tail of the launch instead of forming a launch of their own.
+// 0xFFFFFFFF pads the list to whole blocks.  Block 0 forms the mass factor.
+template <typename T, typename S, int DIM, int BS, bool LOCAL>
+__global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_collide_mixed(
+    CollideArgs<T, S, DIM> a, Phys<T> ph, const uint32_t* __restrict__ list,
+    unsigned int nlist) {
+  if (blockIdx.x == 0) finalize_mass<BS>(a.partials, a.nparts, a.ctrl_w, a.slot_new);
+  const unsigned int w = xcd_block(blockIdx.x, gridDim.x) * (BS / 64) + (threadIdx.x >> 6);
+  if (w >= nlist) return;
+  // one entry per wave: a scalar, so the branch below is uniform
+  const uint32_t c = __builtin_amdgcn_readfirstlane(list[w]);
+  if (c == 0xFFFFFFFFu) return;
+  const uint32_t x = ((c & 0x7FFFFFFFu) << 6) + (threadIdx.x & 63u);
+  if (x >= a.g.N) return;
+  if (c >> 31) collide_node<T, S, DIM, LOCAL, false>(a, ph, x);
+  else collide_node<T, S, DIM, LOCAL, true>(a, ph, x);
+}
+
 // Chunk classification: 1 when all 64 nodes of the chunk exist and are plain
 // fluid.
 template <int BS>

@@ -251,15 +251,16 @@ class AllenCahnSolver:
         return tuple(float(v) for v in out)
 
     def autotune(self, nsteps_each=10):
-        """Pick the fastest launch shape (block size, occupancy cap, chunk split)
-        for the two step kernels on this case.  Advances the solver by
-        18*(nsteps_each+1) steps; returns the block sizes (launch() gives the
+        """Pick the fastest launch shape (block size, occupancy cap, form of
+        pass B) for the two step kernels on this case.  Advances the solver by
+        27*(nsteps_each+1) steps; returns the block sizes (launch() gives the
         rest)."""
         a, b = ctypes.c_int(), ctypes.c_int()
+        k0 = int(nat.lib.lbm_step_count(self._h))
         with torch.cuda.device(self.device):
             nat.check(nat.lib.lbm_autotune(self._h, int(nsteps_each), self._stream(),
                                            ctypes.byref(a), ctypes.byref(b)), "lbm_autotune")
-        n = 18 * (max(int(nsteps_each), 2) + 1)
+        n = int(nat.lib.lbm_step_count(self._h)) - k0
         self.step_count += n
         self.t += float(n)
         self._cache.clear()
@@ -303,8 +304,10 @@ class AllenCahnSolver:
         self.phi_mass_init = m0.value
 
     def set_split(self, split):
-        """Pass B over plain-fluid / wall chunk lists (True) or all nodes."""
-        nat.check(nat.lib.lbm_set_split(self._h, 1 if split else 0), "lbm_set_split")
+        """Form of pass B: 0 (False) one launch over all nodes; 1 (True) two
+        launches over the plain-fluid and the wall chunk lists; 2 one launch
+        over both lists, each XCD's range holding its wall chunks last."""
+        nat.check(nat.lib.lbm_set_split(self._h, int(split)), "lbm_set_split")
 
     def launch(self):
         """(block size A, block size B, cap A, cap B, split) of the step kernels."""

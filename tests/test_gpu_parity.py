@@ -217,14 +217,16 @@ def test_division_by_constants_is_ieee_exact():
 
 
 def test_block_sizes_do_not_change_results():
-    """Every block-size choice of the autotuner gives bit-identical states (the
-    mass reduction tree depends on node indices only)."""
+    """Every launch shape of the autotuner (block sizes, occupancy caps, the
+    three forms of pass B) gives bit-identical states (the mass reduction tree
+    depends on node indices only)."""
     need_gpu()
     print("[gpu] block-size invariance", flush=True)
     ref, z, m = gpu_solver("ridge162_readme_f32", torch.float32)
     ref.step(12)
     for bsa, bsb, ca, cb, sp in ((128, 512, 0, 0, 0), (512, 128, 0, 0, 1), (256, 256, 6, 4, 0),
-                                 (128, 256, 6, 5, 1)):
+                                 (128, 256, 6, 5, 1), (256, 128, 0, 4, 2), (512, 256, 6, 0, 2),
+                                 (128, 512, 0, 5, 2)):
         s, _, _ = gpu_solver("ridge162_readme_f32", torch.float32)
         s.set_block_sizes(bsa, bsb)
         s.set_occupancy(ca, cb)
@@ -241,24 +243,26 @@ def test_autotune_picks_valid_sizes():
     a, b = s.autotune(3)
     assert a in (128, 256, 512) and b in (128, 256, 512)
     la, lb, ca, cb, sp = s.launch()
-    assert (la, lb) == (a, b) and ca in (0, 6) and cb in (0, 4, 5) and sp in (0, 1)
-    assert s.step_count == 72
+    assert (la, lb) == (a, b) and ca in (0, 6) and cb in (0, 4, 5) and sp in (0, 1, 2)
+    assert s.step_count == 27 * 4
 
 
+@pytest.mark.parametrize("split", [1, 2])
 @pytest.mark.parametrize("grid", [(19, 21, 17), (37, 29, 1)])
-def test_ragged_grid_matches_oracle(grid):
+def test_ragged_grid_matches_oracle(grid, split):
     """Node counts that are multiples of neither the 64-node wave nor the
     256-node distribution block (6,783 and 1,073 nodes): the padded tail of
     the last block must stay out of every pull, sum and stencil.  Ridge with
     fraction, impact at theta = 162 with amplification 1.5, README
-    construction; fp64 kernels against the fp64 oracle over 25 steps."""
+    construction; fp64 kernels against the fp64 oracle over 25 steps, with
+    pass B as two chunk-list launches and as the single mixed launch."""
     need_gpu()
     from geometry.substrate import create_substrate_with_fraction
     from lbm.fe_ac_solver import AllenCahnSolver
     from lbm.fe_config import FEConfig
     from lbm.fe_droplet import create_fe_droplet_with_impact
     nx, ny, nz = grid
-    print(f"[gpu] ragged grid {grid}", flush=True)
+    print(f"[gpu] ragged grid {grid} split {split}", flush=True)
     two_d = nz == 1
     centre = (nx / 2.0, 11.0) if two_d else (nx / 2.0, ny / 2.0, 10.0)
     u_imp = (0.0, -0.04) if two_d else (0.0, 0.0, -0.04)
@@ -272,6 +276,7 @@ def test_ragged_grid_matches_oracle(grid):
     s = AllenCahnSolver(cfg, dtype=torch.float64, **solver_kwargs(m, None, "cuda"))
     s.set_solid(solid, frac)
     s.init_fields(C0, u0)
+    s.set_split(split)
     o = OracleSolver(Params(**config_kwargs(m)), dtype=np.float64, **solver_kwargs(m))
     o.set_solid(solid, frac)
     o.init_fields(C0, u0)
