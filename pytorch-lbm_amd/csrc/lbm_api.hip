@@ -398,11 +398,11 @@ int lbm_destroy(lbm_solver* s) {
 // gets a contiguous slice of the plain chunks, then a slice of the wall
 // chunks (bit 31 set) in proportion to its capacity; padding entries are
 // 0xFFFFFFFF.  `list` holds the plain chunks, then the wall chunks.
-static std::vector<uint32_t> mixed_list(const std::vector<uint32_t>& list, unsigned int np,
-                                        unsigned int nw, unsigned int wpb) {
+static bool mixed_list(const std::vector<uint32_t>& list, unsigned int np, unsigned int nw,
+                       unsigned int wpb, std::vector<uint32_t>& mx) {
   const unsigned int n = np + nw, G = (n + wpb - 1) / wpb;
   const unsigned int per = G / kXCD, rem = G % kXCD;
-  std::vector<uint32_t> mx((size_t)G * wpb, 0xFFFFFFFFu);
+  mx.assign((size_t)G * wpb, 0xFFFFFFFFu);
   size_t pp = 0, ww = 0, left = mx.size();
   for (unsigned int j = 0; j < (unsigned int)kXCD; ++j) {
     const size_t start = ((size_t)j * per + (j < rem ? j : rem)) * wpb;
@@ -417,8 +417,9 @@ static std::vector<uint32_t> mixed_list(const std::vector<uint32_t>& list, unsig
     ww += wj;
     left -= cap;
   }
-  // capacity G * wpb >= n and every range is filled before the next: all placed
-  return mx;
+  // capacity G * wpb >= n and every range takes min(its capacity, what is
+  // left): everything is placed
+  return pp == np && ww == nw;
 }
 
 template <typename T, int DIM>
@@ -470,8 +471,10 @@ static int set_geometry_impl(lbm_solver* s, const void* frac, int fdt, hipStream
   s->n_wall = (unsigned int)(nch - s->n_plain);
   HIPCHK(hipMemcpyAsync(s->chunks, list.data(), nch * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
+  std::vector<uint32_t> mx;
   for (int k = 0; k < 3; ++k) {
-    const std::vector<uint32_t> mx = mixed_list(list, s->n_plain, s->n_wall, 2u << k);
+    if (!mixed_list(list, s->n_plain, s->n_wall, 2u << k, mx))
+      return fail(LBM_EHIP, "pass-B chunk list construction failed");
     if (s->mixed[k]) HIPCHK(hipFree(s->mixed[k]));
     s->mixed[k] = nullptr;
     HIPCHK(hipMalloc(&s->mixed[k], mx.size() * 4));
