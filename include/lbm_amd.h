@@ -157,14 +157,15 @@ int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide);
  * No counterpart in the reference. */
 int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide);
 
-/* Form of pass B.  1 (default): two launches over 64-node chunk lists,
- * plain-fluid chunks with a wall-free kernel (fewer registers, more resident
- * waves), the other chunks with the general kernel.  2: one launch over both
- * lists, arranged so that each XCD runs its plain chunks first and its wall
- * chunks last (the wall chunks fill the launch's tail; the kernel carries the
- * general code's registers).  0: one launch over all nodes.  Results are
- * identical in every form; the autotuner picks one.  LBM_EINVAL for other
- * values.  No counterpart in the reference. */
+/* Form of pass B.  1: two launches over 64-node chunk lists, plain-fluid
+ * chunks with a wall-free kernel (fewer registers, more resident waves), the
+ * other chunks with the general kernel (the default for 16-bit and fp64
+ * storage).  2: one launch over both lists, arranged so that each XCD runs
+ * its plain chunks first and its wall chunks last (the wall chunks fill the
+ * launch's tail; the kernel carries the general code's registers; the
+ * default for fp32).  0: one launch over all nodes.  Results are identical in
+ * every form; the autotuner picks one.  LBM_EINVAL for other values.  No
+ * counterpart in the reference. */
 int lbm_set_split(lbm_solver* s, int split);
 
 /* Current launch shape: out5 = {block size A, block size B, cap A, cap B,

@@ -337,6 +337,10 @@ static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipS
   const size_t es = s->esz, Q = s->Q, D = s->dim;
   s->nblocks = nblk(N);
   s->nbA = s->nbB = s->nblocks;
+  // default pass-B form: the single mixed launch for fp32 (its kernel keeps
+  // the plain code's occupancy there), the two chunk-list launches otherwise
+  // (16-bit storage loses occupancy to the general code's registers)
+  s->split = p.precision == LBM_F32 ? 2 : 1;
   hipError_t e = hipSuccess;
 #define ALLOC(ptr, bytes)                                                          \
   if (e == hipSuccess) {                                                           \

@@ -223,6 +223,7 @@ def test_block_sizes_do_not_change_results():
     need_gpu()
     print("[gpu] block-size invariance", flush=True)
     ref, z, m = gpu_solver("ridge162_readme_f32", torch.float32)
+    assert ref.launch()[4] == 2          # fp32 default: the single mixed pass-B launch
     ref.step(12)
     for bsa, bsb, ca, cb, sp in ((128, 512, 0, 0, 0), (512, 128, 0, 0, 1), (256, 256, 6, 4, 0),
                                  (128, 256, 6, 5, 1), (256, 128, 0, 4, 2), (512, 256, 6, 0, 2),
