@@ -141,6 +141,14 @@ int main() {
     lbm_solver* s = nullptr;
     OK(lbm_create(&p, &s));
     CHECK(lbm_step(s, 1, nullptr) == LBM_ESTATE, "step before init accepted");
+    int launch[5] = {0, 0, 0, 0, 0};
+    OK(lbm_get_launch(s, launch));
+    CHECK(launch[4] == 2, "fp32 default pass-B form %d", launch[4]);
+    CHECK(lbm_set_split(s, 3) == LBM_EINVAL && lbm_set_split(s, -1) == LBM_EINVAL,
+          "bad pass-B form accepted");
+    OK(lbm_set_split(s, 1));
+    OK(lbm_get_launch(s, launch));
+    CHECK(launch[4] == 1, "pass-B form not set: %d", launch[4]);
     OK(lbm_destroy(s));
   }
 
