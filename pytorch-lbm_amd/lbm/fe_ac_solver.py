@@ -310,7 +310,8 @@ class AllenCahnSolver:
         nat.check(nat.lib.lbm_set_split(self._h, int(split)), "lbm_set_split")
 
     def launch(self):
-        """(block size A, block size B, cap A, cap B, split) of the step kernels."""
+        """(block size A, block size B, cap A, cap B, form of pass B) of the step
+        kernels (see set_split)."""
         out = (ctypes.c_int * 5)()
         nat.check(nat.lib.lbm_get_launch(self._h, out), "lbm_get_launch")
         return tuple(out)
