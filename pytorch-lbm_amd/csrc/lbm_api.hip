@@ -20,10 +20,12 @@
 
 #include "../../include/lbm_amd.h"
 #include "lbm_kernels.hpp"
+#include "lbm_lists.hpp"
 #include "lbm_geometry.hpp"
 #include "lbm_measure.hpp"
 
 using namespace lbm;
+static_assert(kListXCD == kXCD, "host chunk lists and device block mapping disagree on XCDs");
 
 static thread_local std::string g_err;
 
@@ -394,36 +396,6 @@ int lbm_destroy(lbm_solver* s) {
     if (q) (void)hipFree(q);
   delete s;
   return LBM_OK;
-}
-
-// The single-launch pass-B list for blocks of `wpb` waves: the launch has
-// G = ceil(n / wpb) blocks, and xcd_block gives XCD j the logical blocks
-// [j*(G/8) + min(j, G%8), ...) of G/8 (+1 for j < G%8).  Each XCD's range
-// gets a contiguous slice of the plain chunks, then a slice of the wall
-// chunks (bit 31 set) in proportion to its capacity; padding entries are
-// 0xFFFFFFFF.  `list` holds the plain chunks, then the wall chunks.
-static bool mixed_list(const std::vector<uint32_t>& list, unsigned int np, unsigned int nw,
-                       unsigned int wpb, std::vector<uint32_t>& mx) {
-  const unsigned int n = np + nw, G = (n + wpb - 1) / wpb;
-  const unsigned int per = G / kXCD, rem = G % kXCD;
-  mx.assign((size_t)G * wpb, 0xFFFFFFFFu);
-  size_t pp = 0, ww = 0, left = mx.size();
-  for (unsigned int j = 0; j < (unsigned int)kXCD; ++j) {
-    const size_t start = ((size_t)j * per + (j < rem ? j : rem)) * wpb;
-    const size_t cap = ((size_t)per + (j < rem ? 1 : 0)) * wpb;
-    size_t wj = left ? (size_t)((double)(nw - ww) * (double)cap / (double)left + 0.5) : 0;
-    wj = std::min(std::min(wj, (size_t)nw - ww), cap);
-    size_t pj = std::min(cap - wj, (size_t)np - pp);
-    wj = std::min(cap - pj, (size_t)nw - ww);   // fill what the plain share leaves
-    for (size_t t = 0; t < pj; ++t) mx[start + t] = list[pp + t];
-    for (size_t t = 0; t < wj; ++t) mx[start + pj + t] = list[np + ww + t] | 0x80000000u;
-    pp += pj;
-    ww += wj;
-    left -= cap;
-  }
-  // capacity G * wpb >= n and every range takes min(its capacity, what is
-  // left): everything is placed
-  return pp == np && ww == nw;
 }
 
 template <typename T, int DIM>
