@@ -365,6 +365,8 @@ def main():
                     help="plain: the roofline/MLUPS line (default); precision: storage forms "
                          "against fp64 on the same case; sweep: 48 uneven cases batched vs "
                          "sequential; serve: continuous batching of a Poisson request stream")
+    ap.add_argument("--rate", type=float, default=4.0,
+                    help="serve mode: mean request arrivals per second of the Poisson stream")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a device copy of this many MiB (a known-size kernel "
                          "for scaling HBM byte counters)")
@@ -382,7 +384,7 @@ def main():
         print(json.dumps(sweep_bench()), flush=True)
         return
     if args.mode == "serve":
-        print(json.dumps(serve_bench()), flush=True)
+        print(json.dumps(serve_bench(rate=args.rate)), flush=True)
         return
     print(f"[bench] building case on {torch.cuda.get_device_name(0)}", file=sys.stderr,
           flush=True)
