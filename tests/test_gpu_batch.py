@@ -45,7 +45,7 @@ def test_same_alone_and_in_the_mix(storage):
         assert mix.status(cid) == (s.steps, s.steps)
     for k, (cid, s) in enumerate(zip(ids, sp)):
         alone = BatchEngine(storage)
-        alone.set_split(k % 3)               # every form of pass B: same result
+        alone.set_split(k % 2)               # one launch or the chunk lists: same result
         a = alone.add(s)
         alone.step(s.steps)
         for name in ("phi", "u", "f"):
