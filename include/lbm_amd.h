@@ -137,7 +137,11 @@ int lbm_step(lbm_solver* s, int nsteps, void* stream);
 int lbm_graph_capture(lbm_solver* s, int nsteps);
 
 /* Launch the captured graph on `stream`: advances the solver by the captured
- * number of steps. */
+ * number of steps.  LBM_ESTATE when no graph is held or the state moved since
+ * the capture (an odd number of eager steps, an injection); lbm_set_geometry,
+ * lbm_set_amplification_field, lbm_init_fields, lbm_load_state,
+ * lbm_inject_droplet and the launch-shape setters drop a captured graph
+ * (waiting for the device first, in case a replay is in flight). */
 int lbm_graph_launch(lbm_solver* s, void* stream);
 
 /* Measurement harness: runs `nsteps` steps on `stream` with HIP events
@@ -289,17 +293,39 @@ size_t lbm_case_footprint(const lbm_params* p);
  * the budget.  Does not synchronise other cases' work. */
 int lbm_batch_add(lbm_batch* b, const lbm_case_spec* spec, void* stream, int* case_id);
 
+/* lbm_batch_add with the case's inputs handed over instead of generated:
+ * solid (uint8 0/1), the solid fraction (frac_dtype, nullable) and the initial
+ * phi and velocity (fp32, u nullable: at rest), device arrays in the public
+ * order -- exactly what AllenCahnSolver.set_solid / init_fields take
+ * (lbm/fe_ac_solver.py:193-261).  A case built from the reference's
+ * generators (geometry/substrate.py:162-267, lbm/fe_droplet.py:22-112) then
+ * starts from the same bits as a stand-alone solver.  The spec's grid,
+ * physics and step target are used; its substrate, R_star, R_d, center,
+ * radius and u_impact are not read.  Synchronises `stream`. */
+int lbm_batch_add_fields(lbm_batch* b, const lbm_case_spec* spec, const uint8_t* solid_dev,
+                         const void* frac_dev, int frac_dtype, const void* phi_dev,
+                         const void* u_dev, void* stream, int* case_id);
+
 /* Release a case (synchronises the device). */
 int lbm_batch_remove(lbm_batch* b, int case_id);
 
-/* The solver handle of a case, for lbm_get_field / lbm_measure / ... */
+/* The solver handle of a case, for the calls that only read it
+ * (lbm_get_field, lbm_field_bytes, lbm_measure[_async], lbm_contact_radius,
+ * lbm_get_mass, lbm_step_count, lbm_state_bytes, lbm_save_state,
+ * lbm_grad_lap).  The calls that would change the case behind the batch's
+ * back (lbm_step, lbm_graph_*, lbm_set_geometry, lbm_set_amplification_field,
+ * lbm_init_fields, lbm_load_state, lbm_inject_droplet, the launch-shape
+ * setters, lbm_autotune, lbm_profile_steps, lbm_destroy) return LBM_EINVAL for
+ * such a handle. */
 int lbm_batch_solver(lbm_batch* b, int case_id, lbm_solver** s);
 
 /* Restore a case from an lbm_save_state image of its solver (taken with
  * lbm_batch_solver + lbm_save_state): the case resumes bit for bit at the
  * image's step.  For crash-safe sweep queues (the reference's sweep resume
  * only skips finished cases, scripts/run_pending_sweep.py:207-240).
- * Synchronises `stream`; a captured batch graph must be captured again. */
+ * LBM_EINVAL for an image past the case's step target or one whose buffer set
+ * does not follow its step parity (a stand-alone solver's image taken after
+ * an injection).  Synchronises `stream`; drops a captured batch graph. */
 int lbm_batch_load_state(lbm_batch* b, int case_id, const void* src, void* stream);
 
 /* Steps done and step target of a case. */
@@ -325,7 +351,10 @@ int lbm_batch_autotune(lbm_batch* b, int nsteps_each, void* stream, int* bs_phi,
                        int* bs_collide);
 
 /* Capture nsteps (even) batch steps into a graph for the current case set;
- * every active case must have stepped once and have nsteps left. */
+ * every active case must have stepped once and have nsteps left.  Any change
+ * of the case set or launch shape (add, remove, a case reaching its target,
+ * set_split, set_block_sizes, autotune, load_state) drops the graph, and
+ * lbm_batch_graph_launch then returns LBM_ESTATE until it is captured again. */
 int lbm_batch_graph_capture(lbm_batch* b, int nsteps, void* stream);
 int lbm_batch_graph_launch(lbm_batch* b, void* stream);
 

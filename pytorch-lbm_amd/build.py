@@ -9,14 +9,18 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "lbm_api.hip")
+CSRC = os.path.join(HERE, "csrc")
+SRC = os.path.join(CSRC, "lbm_api.hip")
+INST = os.path.join(CSRC, "lbm_inst.hip")
 OUT_DIR = os.path.join(HERE, "lbm", "_native")
 OUT = os.path.join(OUT_DIR, "liblbm_amd.so")
+OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-FLAGS = [
+# device and host code flags of every translation unit
+CFLAGS = [
     "--offload-arch=gfx950",
-    "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "-fPIC",
     "-mcode-object-version=5",
     # bit-level parity with the reference's separately rounded tensor ops
     "-ffp-contract=off",
@@ -24,6 +28,13 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+FLAGS = CFLAGS + ["-shared"]
+
+# the entry points, plus the launch sequences of each storage type (the bulk
+# of the device code) in objects of their own, compiled in parallel
+UNITS = [("lbm_api", SRC, [])] + [
+    (f"lbm_inst_{name}", INST, [f"-DLBM_INST_PREC={k}"])
+    for k, name in enumerate(("f32", "f64", "bf16", "f16"))]
 
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -31,12 +42,31 @@ DRIVER_SRC = os.path.join(os.path.dirname(HERE), "tests", "c_abi", "abi_driver.c
 DRIVER = os.path.join(os.path.dirname(DRIVER_SRC), "abi_driver")
 
 
-def build(verbose=True):
+def build(verbose=True, jobs=None):
     os.makedirs(OUT_DIR, exist_ok=True)
-    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp", SRC]
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    jobs = jobs or int(os.environ.get("MAX_JOBS", "0")) or min(5, os.cpu_count() or 1)
+    cmds = []
+    for name, src, defs in UNITS:
+        obj = os.path.join(OBJ_DIR, name + ".o")
+        cmds.append((obj, [HIPCC] + CFLAGS + defs + ["-c", "-o", obj, src]))
+    pending = list(cmds)
+    running = []
+    while pending or running:
+        while pending and len(running) < jobs:
+            obj, cmd = pending.pop(0)
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            running.append((cmd, subprocess.Popen(cmd)))
+        cmd, proc = running.pop(0)
+        if proc.wait() != 0:
+            for _, other in running:
+                other.wait()
+            raise subprocess.CalledProcessError(proc.returncode, cmd)
+    link = [HIPCC] + FLAGS + ["-o", OUT + ".tmp"] + [obj for obj, _ in cmds]
     if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+        print(" ".join(link), flush=True)
+    subprocess.run(link, check=True)
     os.replace(OUT + ".tmp", OUT)
     build_driver(verbose)
     return OUT
@@ -61,6 +91,7 @@ def build_driver(verbose=True):
 
 if __name__ == "__main__":
     if "--check" in sys.argv:
-        print(" ".join([HIPCC] + FLAGS + ["-o", OUT, SRC]))
+        for name, src, defs in UNITS:
+            print(" ".join([HIPCC] + CFLAGS + defs + ["-c", "-o", name + ".o", src]))
     else:
         build()

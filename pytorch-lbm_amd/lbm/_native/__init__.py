@@ -105,6 +105,9 @@ _SIGS = {
     "lbm_batch_destroy": (ctypes.c_int, [_vp]),
     "lbm_batch_add": (ctypes.c_int, [_vp, ctypes.POINTER(LbmCaseSpec), _vp,
                                      ctypes.POINTER(ctypes.c_int)]),
+    "lbm_batch_add_fields": (ctypes.c_int, [_vp, ctypes.POINTER(LbmCaseSpec), _vp, _vp,
+                                            ctypes.c_int, _vp, _vp, _vp,
+                                            ctypes.POINTER(ctypes.c_int)]),
     "lbm_batch_remove": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbm_batch_solver": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
     "lbm_batch_load_state": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),

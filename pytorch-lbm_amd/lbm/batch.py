@@ -19,6 +19,7 @@ import math
 from dataclasses import dataclass, field
 from typing import Optional, Tuple
 
+import numpy as np
 import torch
 
 from . import _native as nat
@@ -107,9 +108,25 @@ class BatchEngine:
     def _stream(self):
         return nat.stream_ptr(self.device)
 
-    def add(self, spec: CaseSpec):
-        """Build the case on the device; returns its id.  Raises LbmError with
-        code -4 when it does not fit the HBM budget."""
+    def add(self, spec: CaseSpec, inputs="host"):
+        """Add a case; returns its id.  Raises LbmError with code -4 when it
+        does not fit the HBM budget.
+
+        inputs="host" (default): the substrate and droplet come from the
+        reference's generators (geometry/substrate.py, lbm/fe_droplet.py, run
+        on the host in numpy) and are handed over as arrays, so the case
+        starts from the same bits as an AllenCahnSolver built the same way.
+        inputs="device": the library builds them on the device from the spec
+        (faster for large grids; the droplet's tanh may then differ from
+        numpy's in the last fp32 bit of a few values).  A tuple is taken as
+        host_inputs(spec) prepared in advance."""
+        if isinstance(inputs, tuple):
+            return self._add_host(spec, inputs)
+        if inputs == "host":
+            return self._add_host(spec, host_inputs(spec))
+        if inputs != "device":
+            raise ValueError(f"inputs must be 'host', 'device' or the tuple of "
+                             f"host_inputs(spec), not {inputs!r}")
         cs = nat.LbmCaseSpec()
         cs.p = _params(spec, self.precision)
         cs.substrate = SUBSTRATES[spec.substrate]
@@ -128,6 +145,29 @@ class BatchEngine:
             rc = nat.lib.lbm_batch_add(self._h, ctypes.byref(cs), self._stream(),
                                        ctypes.byref(cid))
         nat.check(rc, "lbm_batch_add")
+        self.cases[cid.value] = spec
+        return cid.value
+
+    def _add_host(self, spec: CaseSpec, arrays):
+        solid, frac, phi0, u0 = arrays
+        dev = self.device
+        cs = nat.LbmCaseSpec()
+        cs.p = _params(spec, self.precision)
+        cs.steps = int(spec.steps)
+        solid_t = torch.from_numpy(solid.astype(np.uint8)).to(dev)
+        fdt = nat.FRAC_NONE
+        frac_t = None
+        if frac is not None:
+            fdt = nat.FRAC_F64 if frac.dtype == np.float64 else nat.FRAC_F32
+            frac_t = torch.from_numpy(np.ascontiguousarray(frac)).to(dev)
+        phi_t = torch.from_numpy(phi0).to(dev)
+        u_t = torch.from_numpy(u0).to(dev)
+        cid = ctypes.c_int()
+        with torch.cuda.device(dev):
+            rc = nat.lib.lbm_batch_add_fields(self._h, ctypes.byref(cs), nat.ptr(solid_t),
+                                              nat.ptr(frac_t), fdt, nat.ptr(phi_t),
+                                              nat.ptr(u_t), self._stream(), ctypes.byref(cid))
+        nat.check(rc, "lbm_batch_add_fields")
         self.cases[cid.value] = spec
         return cid.value
 
@@ -232,6 +272,36 @@ class BatchEngine:
                                              spec.grid[1] / 2.0, ctypes.byref(out),
                                              self._stream()), "lbm_contact_radius")
         return out.value
+
+
+def droplet_center(spec: CaseSpec, solid):
+    """The sweep driver's droplet placement (scripts/run_thesis_sweep.py:
+    146-157) on the solid mask handed to set_solid: centre column (nx/2, ny/2),
+    gap 2 above the substrate top there (the highest rim anywhere for a
+    concave cavity), clamped below the lid; an explicit spec.center wins."""
+    if spec.center is not None:
+        return tuple(float(c) for c in spec.center)
+    nx, ny, nz = spec.grid
+    has = solid.any(axis=2)
+    top = np.where(has, nz - 1 - np.argmax(solid[:, :, ::-1], axis=2), 0)
+    ridge_top = int(top.max()) if spec.substrate == "concave" else int(top[nx // 2, ny // 2])
+    cz = min(ridge_top + 2.0 + spec.R_drop, nz - spec.R_drop - 2)
+    return (nx / 2.0, ny / 2.0, cz)
+
+
+def host_inputs(spec: CaseSpec):
+    """(solid, fraction, phi0, u0) of a case from the reference's generators
+    (geometry/substrate.py:162-267, lbm/fe_droplet.py:22-112), placed by
+    droplet_center()."""
+    from geometry.substrate import create_substrate_with_fraction
+    from .fe_droplet import create_fe_droplet_with_impact
+    nx, ny, nz = spec.grid
+    solid, frac = create_substrate_with_fraction(nx, ny, nz, spec.substrate, spec.R_star,
+                                                 spec.R_drop)
+    c = droplet_center(spec, solid)
+    phi0, _, u0 = create_fe_droplet_with_impact(nx, ny, nz, center=c, radius=spec.R_drop,
+                                                xi=spec.xi, u_impact=(0.0, 0.0, spec.u_impact))
+    return solid, frac, phi0, u0
 
 
 def sweep_cases(seed=2024, n=48, steps=1500):

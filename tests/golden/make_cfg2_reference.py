@@ -12,8 +12,14 @@ R=22.5 at (75,75,80), u=(0,0,-0.05).
 Writes tests/golden/cfg2_ridge162_summary.npz (per checkpoint: sum/sumsq/max of
 phi, sum of |u|^2, extents Dx/Dy/Dz/z_com of phi>0.5 & ~solid, mid-plane
 slices phi[:,75,:] and u_z[:,75,:]) and tests/golden/reference_cpu_timing.json.
+With REF_DTYPE=float64 the same run is made in the reference's fp64 form
+(dtype=torch.float64, the same float32 generator inputs) and written to
+tests/golden/cfg2_ridge162_f64_summary.npz (slices kept in fp64); the
+distance between the two is the reference's own fp32 rounding envelope.
 
-Usage (build container only): python tests/golden/make_cfg2_reference.py [steps]
+Usage (build container only):
+    python tests/golden/make_cfg2_reference.py [steps]
+    REF_DTYPE=float64 python tests/golden/make_cfg2_reference.py [steps]
 """
 import os
 import sys
@@ -56,12 +62,15 @@ def summary(sol):
 def main():
     nsteps = int(sys.argv[1]) if len(sys.argv) > 1 else 2000
     nthreads = int(os.environ.get("REF_THREADS", "6"))
+    dtype = getattr(torch, os.environ.get("REF_DTYPE", "float32"))
+    f64 = dtype == torch.float64
+    fname = "cfg2_ridge162_f64_summary.npz" if f64 else "cfg2_ridge162_summary.npz"
     torch.set_num_threads(nthreads)
     cfg = FEConfig(nx=150, ny=150, nz=107, rho_l=1.0, rho_g=1.0 / 828.0,
                    sigma=0.01424, xi=4.0, tau_l=0.53, tau_g=0.53,
                    theta_eq=162.0, device="cpu", max_steps=2000)
     with contextlib.redirect_stdout(io.StringIO()):
-        sol = AllenCahnSolver(cfg, dtype=torch.float32, geo_amplification=1.5,
+        sol = AllenCahnSolver(cfg, dtype=dtype, geo_amplification=1.5,
                               geometric_wetting=True)
         solid, frac = create_substrate_with_fraction(
             150, 150, 107, substrate_type="ridge", R_star=1.0, R_d=22.5)
@@ -82,11 +91,10 @@ def main():
                 out[f"s{s}_{k}"] = np.array(v)
             out[f"s{s}_phi_xz"] = sol.phi.numpy()[:, 75, :].copy()
             out[f"s{s}_uz_xz"] = sol.u.numpy()[2, :, 75, :].copy()
-            np.savez_compressed(os.path.join(HERE, "cfg2_ridge162_summary.npz"),
-                                **out)
+            np.savez_compressed(os.path.join(HERE, fname), **out)
             print(f"step {s}: {np.mean(times[-min(len(times), 20):]):.3f} s/step",
                   flush=True)
-        if s == 25:
+        if s == 25 and not f64:
             t = np.array(times[5:25])
             rec = {"workload": "ridge impact D3Q19 150x150x107 fp32, README "
                                "construction (tau_h=1.0, boundary_relax=0.3)",

@@ -90,34 +90,138 @@ def test_admission_by_budget():
     assert b.bytes_used() <= budget
 
 
-def test_batch_case_matches_solver_api():
-    """The device-built case equals the AllenCahnSolver case built from the
-    numpy generators (device tanh may differ from numpy's in the last fp32 bit
-    of a few initial values, hence a tolerance)."""
-    need_gpu()
-    from geometry.substrate import create_substrate_with_fraction
-    from lbm.batch import BatchEngine, CaseSpec
+def _solver_case(s, steps, storage=None, dtype=torch.float32):
+    """The same case through AllenCahnSolver, built from the numpy generators
+    and placed by the sweep driver's rule."""
+    from lbm.batch import host_inputs
     from lbm.fe_ac_solver import AllenCahnSolver
-    from lbm.fe_droplet import create_fe_droplet_with_impact
+    cfg = s.config()
+    cfg.device = "cuda"
+    sol = AllenCahnSolver(cfg, dtype=dtype, geo_amplification=s.geo_amplification,
+                          geometric_wetting=s.geometric_wetting,
+                          boundary_relax=s.boundary_relax, storage=storage)
+    solid, frac, phi0, u0 = host_inputs(s)
+    sol.set_solid(solid, frac)
+    sol.init_fields(phi0, u0)
+    sol.step(steps)
+    return sol
+
+
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+def test_batch_case_matches_solver_api(storage):
+    """A batch case built from the host generators equals the AllenCahnSolver
+    case built from the same generators, bit for bit (same inputs, same
+    kernels, its own reduction tree)."""
+    need_gpu()
+    from lbm.batch import BatchEngine, CaseSpec
+    print(f"[gpu] batch case vs solver api {storage}", flush=True)
+    s = CaseSpec(grid=(24, 24, 20), substrate="ridge", R_star=1.0, R_drop=5.0,
+                 theta_eq=162.0, u_impact=-0.05, steps=10)
+    b = BatchEngine(storage)
+    cid = b.add(s)
+    b.step(10)
+    sol = _solver_case(s, 10, storage=None if storage == "fp32" else storage)
+    for name in ("phi", "u"):
+        np.testing.assert_array_equal(getattr(sol, name).cpu().numpy(),
+                                      b.field(cid, name).cpu().numpy())
+
+
+def test_device_built_case_close_to_host_built():
+    """inputs='device' builds substrate and droplet on the device: the same
+    case to the last fp32 bit of a few tanh values (the device tanh is not
+    numpy's)."""
+    need_gpu()
+    from lbm.batch import BatchEngine, CaseSpec
     s = CaseSpec(grid=(24, 24, 20), substrate="ridge", R_star=1.0, R_drop=5.0,
                  theta_eq=162.0, u_impact=-0.05, steps=10)
     b = BatchEngine("fp32")
-    cid = b.add(s)
+    ch = b.add(s)
+    cd = b.add(s, inputs="device")
     b.step(10)
-    cfg = s.config()
-    cfg.device = "cuda"
-    sol = AllenCahnSolver(cfg, dtype=torch.float32, geo_amplification=1.5,
-                          geometric_wetting=True)
-    solid, frac = create_substrate_with_fraction(24, 24, 20, "ridge", 1.0, 5.0)
-    sol.set_solid(solid, frac)
-    # placement rule of scripts/run_thesis_sweep.py:146-157 on the solid that
-    # is handed to set_solid (the volume-penalization solid)
-    top = int(np.nonzero(solid[12, 12])[0].max())
-    cz = min(top + 2.0 + 5.0, 20 - 5.0 - 2)
-    phi0, _, u0 = create_fe_droplet_with_impact(24, 24, 20, center=(12.0, 12.0, cz), radius=5.0,
-                                                xi=4.0, u_impact=(0, 0, -0.05))
-    sol.init_fields(phi0, u0)
-    sol.step(10)
-    a = sol.phi.cpu().numpy()
-    c = b.field(cid, "phi").cpu().numpy()
+    a = b.field(ch, "phi").cpu().numpy()
+    c = b.field(cd, "phi").cpu().numpy()
     assert np.linalg.norm(a - c) / np.linalg.norm(a) < 1e-5
+
+
+def test_stale_batch_graph_is_refused():
+    """A captured batch graph holds the case set's block maps and grid sizes:
+    after the case set or the launch shape changes (and one eager step
+    re-uploads the tables), replaying it must be refused, not run."""
+    need_gpu()
+    from lbm import _native as nat
+    from lbm.batch import BatchEngine
+    sp = specs()
+    for s in sp:
+        s.steps = 40
+    b = BatchEngine("fp32")
+    ids = [b.add(s) for s in sp[:2]]
+    b.step(1)
+    b.capture_graph(4)
+    b.run_graph(1)                       # valid replay
+    b.add(sp[2])
+    b.step(1)
+    with pytest.raises(nat.LbmError):
+        b.run_graph(1)
+    b.capture_graph(2)
+    b.run_graph(1)
+    b.set_split(False)
+    b.step(1)
+    with pytest.raises(nat.LbmError):
+        b.run_graph(1)
+    b.capture_graph(2)
+    nat.check(nat.lib.lbm_batch_set_block_sizes(b._h, 128, 512), "set_block_sizes")
+    b.step(1)
+    with pytest.raises(nat.LbmError):
+        b.run_graph(1)
+    torch.cuda.synchronize()
+    assert all(b.status(c)[0] > 0 for c in ids)
+
+
+def test_batched_solver_handle_refuses_changes():
+    """The solver handle of a batch case serves reads only: calls that would
+    change the case behind the batch's back are refused."""
+    need_gpu()
+    import ctypes
+    from lbm import _native as nat
+    from lbm.batch import BatchEngine
+    b = BatchEngine("fp32")
+    cid = b.add(specs()[0])
+    h = b._solver(cid)
+    st = nat.stream_ptr()
+    assert nat.lib.lbm_step(h, 1, st) == -1
+    assert nat.lib.lbm_graph_capture(h, 2) == -1
+    assert nat.lib.lbm_set_block_sizes(h, 256, 256) == -1
+    assert nat.lib.lbm_destroy(h) == -1
+    n = int(nat.lib.lbm_state_bytes(h))
+    img = torch.empty(n, dtype=torch.uint8, device="cuda")
+    nat.check(nat.lib.lbm_save_state(h, nat.ptr(img), st), "save")      # reads are fine
+    assert nat.lib.lbm_load_state(h, nat.ptr(img), st) == -1
+    out = (ctypes.c_double * 10)()
+    nat.check(nat.lib.lbm_measure(h, out, st), "measure")
+    b.step(2)
+    assert b.status(cid)[0] == 2
+
+
+def test_batch_refuses_image_with_flipped_buffer_set():
+    """A stand-alone solver's image taken right after an injection has its
+    buffer set out of step with the step parity; the batch, whose kernels
+    take the set from the parity, refuses it."""
+    need_gpu()
+    from lbm import _native as nat
+    from lbm.batch import BatchEngine, CaseSpec
+    s = CaseSpec(grid=(24, 24, 20), substrate="ridge", R_star=1.0, R_drop=5.0,
+                 theta_eq=162.0, u_impact=-0.05, steps=30)
+    sol = _solver_case(s, 2)
+    sol.inject_droplet((12.0, 12.0, 8.0), 3.0, 4.0)
+    sol.step(1)                          # the set stays flipped after further steps
+    img = sol.state_dict(device="cpu").pin_memory()
+    b = BatchEngine("fp32")
+    cid = b.add(s)
+    with pytest.raises(nat.LbmError):
+        b.load_case(cid, img)
+    plain = _solver_case(s, 3)           # no injection: set follows the parity
+    b.load_case(cid, plain.state_dict(device="cpu"))
+    assert b.status(cid)[0] == 3
+    b.step(4)
+    plain.step(4)
+    np.testing.assert_array_equal(plain.phi.cpu().numpy(), b.field(cid, "phi").cpu().numpy())

@@ -65,3 +65,50 @@ def test_foreign_image_refused():
     junk = torch.zeros_like(snap)
     with pytest.raises(nat.LbmError):
         a.load_state_dict(junk)
+
+
+def test_stale_solver_graph_is_refused():
+    """A captured step graph reads the buffer set, chunk lists and fields of
+    the moment of capture: after set_solid, init_fields or an odd number of
+    eager steps the replay is refused instead of run on stale pointers; an
+    even number of eager steps brings the state back to the captured parity."""
+    need_gpu()
+    from lbm import _native as nat
+    z, m = load("ridge162_readme_f32")
+    a = make("ridge162_readme_f32")
+    a.step(1)
+    a.capture_graph(2)
+    a.run_graph(1)
+    a.step(1)
+    with pytest.raises(nat.LbmError):
+        a.run_graph(1)
+    a.step(1)
+    a.run_graph(1)                      # parity and buffer set as captured
+    a.set_solid(z["solid"], z["fraction"])
+    a.init_fields(z["phi_init"], z["u_init"])
+    a.step(1)
+    with pytest.raises(nat.LbmError):
+        a.run_graph(1)
+    a.capture_graph(2)
+    a.init_fields(z["phi_init"], z["u_init"])
+    a.step(1)
+    with pytest.raises(nat.LbmError):
+        a.run_graph(1)
+    torch.cuda.synchronize()
+
+
+def test_replay_after_even_eager_steps_matches_eager():
+    """Replays interleaved with an even number of eager steps give the eager
+    trajectory bit for bit."""
+    need_gpu()
+    a = make("ridge162_readme_f32")
+    b = make("ridge162_readme_f32")
+    a.step(1)
+    b.step(1)
+    b.capture_graph(4)
+    for _ in range(3):
+        b.run_graph(1)
+        b.step(2)
+    a.step(18)
+    np.testing.assert_array_equal(a.phi.cpu().numpy(), b.phi.cpu().numpy())
+    np.testing.assert_array_equal(a.u.cpu().numpy(), b.u.cpu().numpy())
