@@ -11,8 +11,10 @@
  *
  * Precision: LBM_F32 and LBM_F64 compute and store in that type (the
  * reference's dtype argument).  LBM_BF16 / LBM_F16 store both distribution
- * sets as 16-bit deviations from the lattice-weight equilibrium and collide
- * in fp32.  Mass reductions are always fp64.
+ * sets in 16 bits and collide in fp32: f as the deviation from the
+ * lattice-weight equilibrium w_i phi, and under LBM_BF16 g as the deviation
+ * from its velocity-carrying part w_i (e_i . u) / cs2 (fp16 stores g as is).
+ * Mass reductions are always fp64.
  *
  * Return value: 0 on success, a negative LBM_E* code otherwise;
  * lbm_last_error() gives a message for the calling thread.
@@ -223,8 +225,9 @@ int lbm_inject_droplet(lbm_solver* s, const void* c_dev, const double* u_impact,
 /* Solver state snapshot for preemption and checkpoints (no counterpart in
  * the reference, which has no solver checkpoint: its sweep resume only skips
  * finished cases, scripts/run_pending_sweep.py:207-240).  The image holds
- * everything step() reads and the observables need -- distributions, velocity,
- * both phi buffers, mass factors, step count -- but not the geometry: restore
+ * everything step() reads and the observables need -- distributions, both
+ * velocity and both phi buffers, mass factors, step count -- but not the
+ * geometry: restore
  * into a solver of the same grid and precision after the same
  * lbm_set_geometry.  `dst` / `src` may be device or host memory of
  * lbm_state_bytes() bytes.  Both synchronise `stream`.  lbm_load_state
@@ -251,6 +254,14 @@ int lbm_measure_async(lbm_solver* s, double* out_dev, void* stream);
  * phi > 0.5 in the first non-solid layer above the substrate at each column
  * (3D only).  Synchronises `stream`. */
 int lbm_contact_radius(lbm_solver* s, double cx, double cy, double* out, void* stream);
+
+/* Base radius of the drop: max horizontal distance from (cx, cy) of liquid
+ * nodes (phi > 0.5, not solid) in the lowest layer that holds liquid (3D
+ * only; 0 without liquid).  The reference's 3D contact-angle estimate takes
+ * r_contact on the first fluid layer above the wall (lbm/fe_measure.py:
+ * 188-241); a drop that spreads on a gas cushion leaves that layer empty, and
+ * this radius follows the drop's own base instead.  Synchronises `stream`. */
+int lbm_base_radius(lbm_solver* s, double cx, double cy, double* out, void* stream);
 
 /* Self-check of the kernels' division by a constant: counts the fp32 inputs
  * (all 2^32 bit patterns, NaNs excepted) whose quotient by d differs from IEEE

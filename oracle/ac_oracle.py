@@ -290,8 +290,12 @@ def cubic_boundary_phi(phi, bc, nc, nfluid, a):
 
 # ---------------------------------------------------------------------------
 # 16-bit storage model (the build's precision extension; no reference
-# counterpart): f* is stored as f*_i(y) - w_i phi(y + e_i) and g* as is, both
-# rounded to nearest-even in bf16 or fp16, then reconstructed in fp32.
+# counterpart): f* is stored as f*_i(y) - w_i phi(y + e_i), rounded to
+# nearest-even in bf16 or fp16, then reconstructed in fp32.  g* is stored as
+# is in fp16; in bf16 as g*_i(y) - (3 w_i)(e_i . u(y)) with the writing
+# node's new velocity (the velocity-carrying odd term of g_eq, which bf16's
+# 8-bit mantissa would otherwise round away), and the same reference is added
+# back when the slot is read.
 # ---------------------------------------------------------------------------
 def round_bf16(a):
     a = np.ascontiguousarray(a, dtype=np.float32)
@@ -504,6 +508,28 @@ class OracleSolver:
         s = self._wv() * ((ec + (ec * ec) / 2.0) - usq[None] / (2.0 * CS2))
         return (P[None] * self._wv() + s).astype(self.dt)
 
+    def _gref(self, u):
+        """(3 w_i) (e_i . u) in fp32 as the kernels form the bf16 g reference:
+        e_i . u summed over the nonzero components in index order."""
+        ref = np.zeros((self.Q,) + u.shape[1:], dtype=np.float32)
+        for i in range(self.Q):
+            acc = None
+            for d in range(self.D):
+                e = self.e[i][d]
+                if e != 0:
+                    t = u[d] if e > 0 else -u[d]
+                    acc = t if acc is None else acc + t
+            if acc is not None:
+                ref[i] = np.float32(3.0 * float(self.w[i])) * acc
+        return ref
+
+    def _store_g(self, g, u):
+        """g as stored (16-bit) and read back; see the storage model above."""
+        if self.storage == 'bf16':
+            ref = self._gref(u)
+            return round16(g - ref, 'bf16') + ref
+        return round16(g, self.storage)
+
     @staticmethod
     def _qsum(a):
         """Sum over the direction axis in the order torch's CPU reduction uses
@@ -537,7 +563,7 @@ class OracleSolver:
         if self.storage is not None:
             wv = self._wv()
             self.f = round16(self.f - wv * phi[None], self.storage) + wv * phi[None]
-            self.g = round16(self.g, self.storage)
+            self.g = self._store_g(self.g, u)
         self.phi_u_prev = None
 
     def inject_droplet(self, center, radius, xi, u_impact=None):
@@ -577,7 +603,7 @@ class OracleSolver:
         if self.storage is not None:
             wv = self._wv()
             f = round16(f - wv * phi[None], self.storage) + wv * phi[None]
-            g = round16(g, self.storage)
+            g = self._store_g(g, u)
         self.phi = phi
         self.rho = (phi * self.p.rho_l + (1.0 - phi) * self.p.rho_g).astype(dt)
         self.u = u
@@ -699,7 +725,7 @@ class OracleSolver:
             for i in range(self.Q):
                 ref = self.w[i] * neighbor(phi, self.e[i])
                 f[i] = round16(f[i] - ref, self.storage) + ref
-            g = round16(g, self.storage)
+            g = self._store_g(g, u)
         st.update(P=P, sigma=sig, F=F, u=u, Fsrc=Fsrc, fstar=f.copy(), gstar=g.copy())
         # S9-S10b: stream (pull from x - e_i), bounce-back, volume penalization
         fs, gs = f, g
@@ -807,3 +833,20 @@ def contact_radius(phi, solid, cx, cy):
     dy = Y.astype(np.float64) - cy
     r2 = dx * dx + dy * dy
     return float(np.sqrt(r2[liquid].max()))
+
+
+def base_radius(phi, solid, cx, cy):
+    """Base radius of the drop: the largest horizontal distance from (cx, cy)
+    of a liquid node (phi > 0.5, not solid) on the lowest layer (last axis)
+    that holds liquid; 0 without liquid.  3D only.  The reference's 3D
+    contact-angle estimate takes its radius on the first fluid layer above the
+    wall (lbm/fe_measure.py:188-241); this follows the drop's own base."""
+    liquid = (phi > 0.5) & ~solid
+    if not liquid.any():
+        return 0.0
+    zs = np.nonzero(liquid.any(axis=(0, 1)))[0]
+    layer = liquid[:, :, zs[0]]
+    xs, ys = np.nonzero(layer)
+    dx = xs.astype(np.float64) - cx
+    dy = ys.astype(np.float64) - cy
+    return float(np.sqrt((dx * dx + dy * dy).max()))

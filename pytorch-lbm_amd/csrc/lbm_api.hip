@@ -112,7 +112,8 @@ static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipS
     ALLOC(s->phi[b], N * es);
     ALLOC(s->dist[b], N);
   }
-  ALLOC(s->u, D * N * es);
+  ALLOC(s->u[0], D * N * es);
+  ALLOC(s->u[1], D * N * es);
   ALLOC(s->eps, N * es);
   ALLOC(s->meta, N);
   ALLOC(s->bnd, N);
@@ -142,7 +143,7 @@ static int create_impl(const lbm_params* pin, lbm_solver** out, bool async, hipS
 static int destroy_impl(lbm_solver* s) {
   if (!s) return LBM_OK;
   drop_graph(s);
-  void* ptrs[] = {s->Fb[0], s->Fb[1], s->Gb[0], s->Gb[1], s->u, s->phi[0], s->phi[1],
+  void* ptrs[] = {s->Fb[0], s->Fb[1], s->Gb[0], s->Gb[1], s->u[0], s->u[1], s->phi[0], s->phi[1],
                   s->eps, s->ampf, s->meta, s->bnd, s->dist[0], s->dist[1], s->solid,
                   s->wmask, s->normal, s->surf, s->partials, s->mparts, s->dout, s->ctrl,
                   s->chunks, s->chunk_flag, s->keep, s->nsw, s->mu,
@@ -641,8 +642,10 @@ int lbm_inject_droplet(lbm_solver* s, const void* c_dev, const double* u_impact,
 //   [0, 256)   StateHeader
 //   then       F[cur], G[cur], G[cur^1] (AoSoA with the header's block size,
 //              the step's distributions and
-//              the previous g set the P observable is pulled from), u,
-//              phi[0], phi[1], Ctrl, kept P, kept mu
+//              the previous g set the P observable is pulled from),
+//              u[0], u[1] (by step parity: the last and the previous step's
+//              velocity, which bf16 g slots are read against), phi[0],
+//              phi[1], Ctrl, kept P, kept mu
 // each array starting at a 256-byte boundary.  The kept P and mu (observables
 // right after a droplet injection) are copied only when the header says so.  Geometry is not included: the
 // restoring solver gets the same set_geometry first.
@@ -656,12 +659,14 @@ struct StateHeader {
   uint64_t bytes;
 };
 constexpr uint32_t kStateMagic = 0x4c424d53u;   // "LBMS"
-constexpr uint32_t kStateVersion = 3;
+constexpr uint32_t kStateVersion = 4;
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
+constexpr int kStateArrays = 10;   // the last two only after an injection
+
 struct StateLayout {
-  size_t dist, u, phi, off[9], total;
+  size_t dist, u, phi, off[kStateArrays], total;
 };
 
 StateLayout state_layout(const lbm_solver* s) {
@@ -669,9 +674,10 @@ StateLayout state_layout(const lbm_solver* s) {
   L.dist = (size_t)s->Q * (size_t)padded_nodes(s->N) * s->dsz;
   L.u = (size_t)s->dim * (size_t)s->N * s->esz;
   L.phi = (size_t)s->N * s->esz;
-  const size_t sz[9] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl), L.phi, L.phi};
+  const size_t sz[kStateArrays] = {L.dist, L.dist, L.dist, L.u, L.u, L.phi, L.phi,
+                                   sizeof(Ctrl), L.phi, L.phi};
   size_t o = 256;
-  for (int i = 0; i < 9; ++i) {
+  for (int i = 0; i < kStateArrays; ++i) {
     L.off[i] = o;
     o += align256(sz[i]);
   }
@@ -697,10 +703,12 @@ int lbm_save_state(lbm_solver* s, void* dst, void* stream) {
   h.block = kAB;
   h.bytes = L.total;
   char* d = (char*)dst;
-  const void* src[9] = {s->Fb[s->cur], s->Gb[s->cur], s->Gb[s->cur ^ 1], s->u, s->phi[0],
-                        s->phi[1], s->ctrl, s->keep, (const char*)s->keep + L.phi};
-  const size_t sz[9] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl), L.phi, L.phi};
-  for (int i = 0; i < (s->kept ? 9 : 7); ++i)
+  const void* src[kStateArrays] = {s->Fb[s->cur], s->Gb[s->cur], s->Gb[s->cur ^ 1], s->u[0],
+                                   s->u[1], s->phi[0], s->phi[1], s->ctrl, s->keep,
+                                   (const char*)s->keep + L.phi};
+  const size_t sz[kStateArrays] = {L.dist, L.dist, L.dist, L.u, L.u, L.phi, L.phi,
+                                   sizeof(Ctrl), L.phi, L.phi};
+  for (int i = 0; i < (s->kept ? kStateArrays : kStateArrays - 2); ++i)
     HIPCHK(hipMemcpyAsync(d + L.off[i], src[i], sz[i], hipMemcpyDefault, st));
   // the header comes from host memory that goes out of scope: synchronous copy
   HIPCHK(hipStreamSynchronize(st));
@@ -732,10 +740,11 @@ static int load_state_internal(lbm_solver* s, const void* src, hipStream_t st) {
   if (h.cur < 0 || h.cur > 1) return fail(LBM_EINVAL, "corrupt state image");
   if (h.kept && !s->keep) HIPCHK(hipMalloc(&s->keep, 3 * L.phi));
   const char* d = (const char*)src;
-  void* dstp[9] = {s->Fb[h.cur], s->Gb[h.cur], s->Gb[h.cur ^ 1], s->u, s->phi[0], s->phi[1],
-                   s->ctrl, s->keep, (char*)s->keep + L.phi};
-  const size_t sz[9] = {L.dist, L.dist, L.dist, L.u, L.phi, L.phi, sizeof(Ctrl), L.phi, L.phi};
-  for (int i = 0; i < (h.kept ? 9 : 7); ++i)
+  void* dstp[kStateArrays] = {s->Fb[h.cur], s->Gb[h.cur], s->Gb[h.cur ^ 1], s->u[0], s->u[1],
+                              s->phi[0], s->phi[1], s->ctrl, s->keep, (char*)s->keep + L.phi};
+  const size_t sz[kStateArrays] = {L.dist, L.dist, L.dist, L.u, L.u, L.phi, L.phi,
+                                   sizeof(Ctrl), L.phi, L.phi};
+  for (int i = 0; i < (h.kept ? kStateArrays : kStateArrays - 2); ++i)
     HIPCHK(hipMemcpyAsync(dstp[i], d + L.off[i], sz[i], hipMemcpyDefault, st));
   s->k = h.k;
   s->cur = h.cur;
@@ -805,6 +814,36 @@ static int contact_impl(lbm_solver* s, double cx, double cy, double* out, hipStr
   return LBM_OK;
 }
 
+template <typename T, int DIM>
+static int base_impl(lbm_solver* s, double cx, double cy, double* out, hipStream_t st) {
+  const Phys<T> ph = make_phys<T, DIM>(s);
+  const unsigned int nb = (unsigned int)((s->N + (long long)BS * kMeasPer - 1) / ((long long)BS * kMeasPer));
+  const int slot = (int)(s->k & 1), scaled = s->local ? 0 : 1;
+  int* zmin = (int*)(s->dout + 14);
+  k_measure<T, DIM, BS><<<nb, BS, 0, st>>>(s->g, ph, (const T*)s->phi[slot], s->meta, s->ctrl,
+                                           slot, scaled, (MeasRec*)s->mparts);
+  k_measure_final<DIM, BS><<<1, BS, 0, st>>>((const MeasRec*)s->mparts, nb, s->dout, zmin);
+  const long long ncol = (long long)s->g.nx * s->g.ny;
+  // the records are consumed: their buffer takes the per-block maxima
+  k_base<T, DIM, BS><<<nblk(ncol), BS, 0, st>>>(s->g, ph, (const T*)s->phi[slot], s->meta,
+                                                s->ctrl, slot, scaled, cx, cy, zmin,
+                                                (double*)s->mparts);
+  k_contact_final<BS><<<1, BS, 0, st>>>((const double*)s->mparts, nblk(ncol), s->dout + 12);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->dout + 12, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_base_radius(lbm_solver* s, double cx, double cy, double* out, void* stream) {
+  if (!s || !out) return fail(LBM_EINVAL, "null argument");
+  if (!s->inited) return fail(LBM_ESTATE, "fields not initialised");
+  if (s->dim != 3) return fail(LBM_EINVAL, "base radius is defined for 3D grids");
+#define CALL(T, D) base_impl<T, D>(s, cx, cy, out, (hipStream_t)stream)
+  return DISPATCH(s, CALL);
+#undef CALL
+}
+
 int lbm_contact_radius(lbm_solver* s, double cx, double cy, double* out, void* stream) {
   if (!s || !out) return fail(LBM_EINVAL, "null argument");
   if (!s->inited) return fail(LBM_ESTATE, "fields not initialised");
@@ -820,7 +859,7 @@ static size_t case_bytes(const lbm_params& p) {
   const size_t es = p.precision == LBM_F64 ? 8 : 4;
   const size_t ds = p.precision == LBM_F64 ? 8 : (p.precision == LBM_F32 ? 4 : 2);
   const size_t nb = (N + 255) / 256;
-  return 4 * Q * (size_t)padded_nodes((long long)N) * ds + 2 * N * es + 2 * N + dim * N * es + N * es + 3 * N + 4 * N +
+  return 4 * Q * (size_t)padded_nodes((long long)N) * ds + 2 * N * es + 2 * N + 2 * dim * N * es + N * es + 3 * N + 4 * N +
          dim * N * 4 + (size_t)p.nx * p.ny * 2 + ((N + 63) / 64 + 1) * 8 +
          nb * sizeof(MeasRec) + 128 + sizeof(Ctrl);
 }

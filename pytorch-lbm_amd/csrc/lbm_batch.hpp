@@ -20,7 +20,7 @@ template <typename T, typename S> struct BatchCase {
   Phys<T> ph;
   S* F[2];
   S* G[2];
-  T* u;
+  T* u[2];                     // velocity by step parity
   T* phi[2];
   T* eps;
   T* ampf;
@@ -92,7 +92,8 @@ __global__ __launch_bounds__(BS) void k_collide_batch(const BatchCase<T, S>* __r
   a.Gin = cs.G[pslot];
   a.Fout = cs.F[nslot];
   a.Gout = cs.G[nslot];
-  a.u = cs.u;
+  a.u_in = cs.u[pslot];
+  a.u_out = cs.u[nslot];
   a.phi_cur = cs.phi[nslot];
   a.phi_prev = cs.phi[pslot];
   a.meta = cs.meta;
@@ -141,7 +142,8 @@ __global__ __launch_bounds__(BS) void k_collide_batch_chunks(
   a.Gin = cs.G[pslot];
   a.Fout = cs.F[nslot];
   a.Gout = cs.G[nslot];
-  a.u = cs.u;
+  a.u_in = cs.u[pslot];
+  a.u_out = cs.u[nslot];
   a.phi_cur = cs.phi[nslot];
   a.phi_prev = cs.phi[pslot];
   a.meta = cs.meta;

@@ -54,7 +54,7 @@ struct lbm_solver {
   bool corr_on = false;
   void* Fb[2] = {nullptr, nullptr};
   void* Gb[2] = {nullptr, nullptr};
-  void* u = nullptr;
+  void* u[2] = {nullptr, nullptr};   // velocity by step parity: after k steps in u[k & 1]
   void* phi[2] = {nullptr, nullptr};
   void* eps = nullptr;
   void* ampf = nullptr;
@@ -160,6 +160,7 @@ template <typename T, int DIM> inline Phys<T> make_phys(const lbm_solver* s) {
     ph.woc[c] = ph.wc[c] / ph.cs2;                   // self._w / self.cs2
     ph.lc[c] = T(2) * ph.woc[c];                     // 2.0 * wi
     ph.lic[c] = T(2) * ph.wc[c];                     // 2.0 * self._w[i]
+    ph.w3c[c] = TT<T>(3.0 * (double)w32<DIM>(i));
   }
   ph.four_beta = TT<T>(4.0 * p.beta);
   ph.kappa = TT<T>(p.kappa);
@@ -329,7 +330,7 @@ template <typename T, typename S, int DIM>
 int init_impl(lbm_solver* s, const void* phi, const void* u, hipStream_t st) {
   const Phys<T> ph = make_phys<T, DIM>(s);
   k_init<T, S, DIM, BS><<<s->nblocks, BS, 0, st>>>(s->g, ph, (const T*)phi, (const T*)u,
-                                                   (T*)s->phi[0], (T*)s->u, (S*)s->Fb[0],
+                                                   (T*)s->phi[0], (T*)s->u[0], (S*)s->Fb[0],
                                                    (S*)s->Gb[0], s->partials);
   k_sum_partials<BS><<<1, BS, 0, st>>>(s->partials, (unsigned int)((s->N + 63) / 64),
                                        &s->ctrl->M0);
@@ -370,7 +371,8 @@ int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev = nullpt
     a.Gin = (const S*)s->Gb[src];
     a.Fout = (S*)s->Fb[dst];
     a.Gout = (S*)s->Gb[dst];
-    a.u = (T*)s->u;
+    a.u_in = (const T*)s->u[pslot];
+    a.u_out = (T*)s->u[nslot];
     a.phi_cur = (const T*)s->phi[nslot];
     a.phi_prev = (const T*)s->phi[pslot];
     a.meta = s->meta;
@@ -509,13 +511,17 @@ int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st,
       return to_public(s, a.phi_cur, dst, s->esz, 1, st);
     case LBM_FIELD_RHO: a.kind = OBS_RHO; break;
     case LBM_FIELD_U:
-      return to_public(s, s->u, dst, s->esz, s->dim, st);
+      return to_public(s, s->u[s->k & 1], dst, s->esz, s->dim, st);
     case LBM_FIELD_MU:
       a.kind = OBS_MU;
       a.init_lap = s->k == 0 ? 1 : 0;
       break;
     case LBM_FIELD_F: a.kind = OBS_F; a.Fsrc = (const S*)s->Fb[s->cur]; break;
-    case LBM_FIELD_G: a.kind = OBS_G; a.Fsrc = (const S*)s->Gb[s->cur]; break;
+    case LBM_FIELD_G:
+      a.kind = OBS_G;
+      a.Fsrc = (const S*)s->Gb[s->cur];
+      a.u_src = (const T*)s->u[s->k & 1];
+      break;
     case LBM_FIELD_P:
       if (s->k == 0) {
         HIPCHK(hipMemsetAsync(dst, 0, N * s->esz, st));
@@ -523,6 +529,7 @@ int get_field_impl(lbm_solver* s, int which, void* dst, hipStream_t st,
       }
       a.kind = OBS_P;
       a.Fsrc = (const S*)s->Gb[s->cur ^ 1];
+      a.u_src = (const T*)s->u[(s->k + 1) & 1];   // the velocity of step k - 1
       a.local = s->prev_local ? 1 : 0;
       break;
     case LBM_FIELD_WALL_NORMAL:
@@ -568,7 +575,10 @@ int inject_impl(lbm_solver* s, const void* c_pub, const double* uimp, hipStream_
   a.pref = (const T*)s->phi[pslot];
   a.C = (const T*)(kp + 2 * fb);
   a.P = (const T*)kp;
-  a.u = (T*)s->u;
+  // the new velocity goes to the spare buffer (neighbours still read the old
+  // one for the bf16 g references) and is copied over after the pass
+  a.u_in = (const T*)s->u[pslot];
+  a.u_out = (T*)s->u[oslot];
   // neighbours read phi[pslot] as their 16-bit reference: the new phi goes
   // to the other slot first and is copied over after the pass
   a.phi_out = (T*)s->phi[oslot];
@@ -586,6 +596,8 @@ int inject_impl(lbm_solver* s, const void* c_pub, const double* uimp, hipStream_
                                        &s->ctrl->M0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(s->phi[pslot], s->phi[oslot], fb, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(s->u[pslot], s->u[oslot], (size_t)s->dim * fb, hipMemcpyDeviceToDevice,
+                        st));
   return LBM_OK;
 }
 
@@ -654,7 +666,8 @@ int batch_upload(lbm_batch* b, hipStream_t st) {
       d.G[i] = (S*)s->Gb[i];
       d.phi[i] = (T*)s->phi[i];
     }
-    d.u = (T*)s->u;
+    d.u[0] = (T*)s->u[0];
+    d.u[1] = (T*)s->u[1];
     d.eps = (T*)s->eps;
     d.ampf = (T*)s->ampf;
     d.meta = s->meta;

@@ -365,6 +365,70 @@ __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, con
   }
 }
 
+// ---------------------------------------------------------------------------
+// bf16 g storage.  u is carried by the odd part of g (u = sum e_i g_i plus the
+// force term), which bf16's 8-bit mantissa rounds away, so a bf16 g slot
+// written at node y for direction i holds g*_i(y) - r_i(y), with the odd
+// velocity term of g_eq, r_i(y) = (3 w_i) (e_i . u(y)), formed from y's new
+// velocity (e_i . u summed over the nonzero components in index order).  A
+// reader adds back the same value, formed from the velocity of the node that
+// wrote the slot: the pull source for fluid nodes, the node itself for
+// bounce-back and penalization slots and in the post-stream ("local") form.
+// fp16 (10-bit mantissa) stores g as is.
+// ---------------------------------------------------------------------------
+template <typename S> constexpr bool kGRef = std::is_same<S, __hip_bfloat16>::value;
+
+// r_i(y) from the velocity field bU (DIM arrays of N32 values).
+template <typename T, int DIM>
+__device__ __forceinline__ T gref(const Phys<T>& ph, int i, const Buf& bU, uint32_t N32,
+                                  uint32_t y) {
+  T v[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) v[d] = E<DIM>(i, d) != 0 ? bget<T, T>(bU, N32, d, y) : T(0);
+  return ph.template w3<DIM>(i) * edot<DIM>(i, v);
+}
+
+// Pull of the g set at node x (the pull() rules), with the bf16 references
+// added back when GREF.  bU: the velocity the slots were written with.
+template <typename T, typename S, int DIM, bool GREF, bool LOCAL, int CP = 0>
+__device__ __forceinline__ void pull_g(const Buf& G, const Buf& bU, uint32_t N32, uint32_t x,
+                                       const Nbrs<DIM>& nb, uint8_t m,
+                                       const T* __restrict__ eps_arr, const Phys<T>& ph,
+                                       T (&g)[Lat<DIM>::Q]) {
+  constexpr int Q = Lat<DIM>::Q;
+  if constexpr (!GREF) {
+    T unused[Q];
+    pull<T, S, DIM, false, false, LOCAL, CP>(G, G, x, nb, m, eps_arr, G, ph, g, unused);
+  } else {
+    const uint32_t vx = dvoff<Q, S>(x);
+    if constexpr (LOCAL) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) g[i] = dload<Q, T, S>(G, i, vx) + gref<T, DIM>(ph, i, bU, N32, x);
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) g[i] = dload<Q, T, S, CP>(G, i, dvoff<Q, S>(nb.i[OPP<DIM>(i)]));
+    __builtin_amdgcn_sched_barrier(0);
+    if (m & M_SOLID) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i)
+        g[i] = dload<Q, T, S>(G, OPP<DIM>(i), vx) + gref<T, DIM>(ph, OPP<DIM>(i), bU, N32, x);
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) g[i] = g[i] + gref<T, DIM>(ph, i, bU, N32, nb.i[OPP<DIM>(i)]);
+    if (m & M_PARTIAL) {
+      const T eps = eps_arr[x];
+      const T a = T(1) - eps;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const T og = dload<Q, T, S>(G, OPP<DIM>(i), vx) + gref<T, DIM>(ph, OPP<DIM>(i), bU, N32, x);
+        g[i] = a * g[i] + eps * og;
+      }
+    }
+  }
+}
+
 template <int DIM, typename S> __device__ __forceinline__ Buf dist_buf(const S* F, long long N) {
   return Buf(F, (long long)Lat<DIM>::Q * padded_nodes(N) * (long long)sizeof(S));
 }
@@ -545,8 +609,9 @@ __global__ __launch_bounds__(BS) void k_init(Grid g, Phys<T> ph, const T* __rest
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       T fe = feq_of<T, DIM>(ph, i, p, ec[i], usq_t);
-      const T ge = geq_of<T, DIM>(ph, i, T(0), ec[i], usq_t);
+      T ge = geq_of<T, DIM>(ph, i, T(0), ec[i], usq_t);
       if constexpr (DEV) fe = fe - ph.template w<DIM>(i) * p;
+      if constexpr (kGRef<S>) ge = ge - ph.template w3<DIM>(i) * edot<DIM>(i, uu);
       dput<Q, T, S>(bF, i, vx, fe);
       dput<Q, T, S>(bG, i, vx, ge);
     }
@@ -786,7 +851,8 @@ template <typename T, typename S, int DIM> struct CollideArgs {
   const S* __restrict__ Gin;
   S* __restrict__ Fout;
   S* __restrict__ Gout;
-  T* __restrict__ u;
+  const T* __restrict__ u_in;   // velocity of the last step (by step parity)
+  T* __restrict__ u_out;        // velocity of this step (the other buffer)
   const T* __restrict__ phi_cur;
   const T* __restrict__ phi_prev;
   const uint8_t* __restrict__ meta;
@@ -835,12 +901,12 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   const uint32_t N32 = (uint32_t)N;
   const Buf bFi = dist_buf<DIM>(a.Fin, N), bGi = dist_buf<DIM>(a.Gin, N);
   const Buf bPp = field_buf(a.phi_prev, N), bPc = field_buf(a.phi_cur, N);
-  const Buf bU = field_buf(a.u, N, DIM);
+  const Buf bU = field_buf(a.u_in, N, DIM), bUo = field_buf(a.u_out, N, DIM);
   // 16-bit storage (VALU-bound): a wave issuing its pulls runs at raised
   // priority, so its loads leave early while other waves compute (measured
   // 142.1 -> 140.6 us/step for bf16; no gain for 4-byte storage)
   if constexpr (DEV) __builtin_amdgcn_s_setprio(1);
-  pull<T, S, DIM, false, false, LOCAL, CP_G>(bGi, bGi, x, nb, m, a.eps, bPp, ph, gg, unused);
+  pull_g<T, S, DIM, kGRef<S>, LOCAL, CP_G>(bGi, bU, N32, x, nb, m, a.eps, ph, gg);
   if constexpr (DEV) __builtin_amdgcn_s_setprio(0);
   T pprev = T(0);
   if (((m & M_TOP) && !LOCAL) || a.phiu_valid) pprev = a.phi_prev[x];
@@ -1005,12 +1071,13 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const T ge = geq_of<T, DIM>(ph, i, P, ec[i], usq_n);
-      const T gs = (gg[i] - omg * (gg[i] - ge)) + G[i];
+      T gs = (gg[i] - omg * (gg[i] - ge)) + G[i];
+      if constexpr (kGRef<S>) gs = gs - ph.template w3<DIM>(i) * edot<DIM>(i, un);
       dput<Q, T, S, CP_G>(bGo, i, vx, gs);
     }
   }
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) bput<T, T>(bU, N32, d, x, un[d]);
+  for (int d = 0; d < DIM; ++d) bput<T, T>(bUo, N32, d, x, un[d]);
 
   // f: pull, mass factor and boundary relax of the previous step, collision
   // with the interface source (odd in e as well)
@@ -1130,6 +1197,7 @@ template <typename T, typename S, int DIM> struct ObsArgs {
   const uint32_t* __restrict__ wmask;
   const float* __restrict__ normal;
   const T* __restrict__ eps;
+  const T* __restrict__ u_src;  // velocity the Fsrc slots were written with (bf16 g)
   const Ctrl* ctrl;
   T* __restrict__ out;
   int kind;
@@ -1191,8 +1259,10 @@ __global__ __launch_bounds__(BS) void k_observe(ObsArgs<T, S, DIM> a, Phys<T> ph
         if (a.local) pull<T, S, DIM, DV, false, true>(bF, bF, x, nb, m, a.eps, bR, ph, f, unused);
         else pull<T, S, DIM, DV, false, false>(bF, bF, x, nb, m, a.eps, bR, ph, f, unused);
       } else {
-        if (a.local) pull<T, S, DIM, false, false, true>(bF, bF, x, nb, m, a.eps, bR, ph, f, unused);
-        else pull<T, S, DIM, false, false, false>(bF, bF, x, nb, m, a.eps, bR, ph, f, unused);
+        const Buf bU = field_buf(a.u_src, N, DIM);
+        const uint32_t N32 = (uint32_t)N;
+        if (a.local) pull_g<T, S, DIM, kGRef<S>, true>(bF, bU, N32, x, nb, m, a.eps, ph, f);
+        else pull_g<T, S, DIM, kGRef<S>, false>(bF, bU, N32, x, nb, m, a.eps, ph, f);
       }
       if (a.kind == OBS_P) {
         a.out[ex] = qsum<Q>(f);
@@ -1259,7 +1329,8 @@ template <typename T, typename S, int DIM> struct InjectArgs {
   const T* __restrict__ pref;   // start-of-step phi of the last step (16-bit f reference)
   const T* __restrict__ C;      // new drop indicator, internal order
   const T* __restrict__ P;      // pressure of the last step, internal order
-  T* __restrict__ u;            // velocity, updated in place
+  const T* __restrict__ u_in;   // velocity of the last step
+  T* __restrict__ u_out;        // velocity after the injection (spare buffer)
   T* __restrict__ phi_out;
   const uint8_t* __restrict__ meta;
   const T* __restrict__ eps;
@@ -1282,9 +1353,10 @@ __global__ __launch_bounds__(BS) void k_inject(InjectArgs<T, S, DIM> a, Phys<T> 
     const Coord<DIM> c = decompose<DIM>(g, x);
     const Nbrs<DIM> nb = neighbours<DIM>(g, x, c);
     const Buf bF = dist_buf<DIM>(a.Fin, N), bG = dist_buf<DIM>(a.Gin, N);
-    const Buf bR = field_buf(a.pref, N);
-    T f[Q], gg[Q];
-    pull<T, S, DIM, DEV, true, LOCAL>(bF, bG, x, nb, m, a.eps, bR, ph, f, gg);
+    const Buf bR = field_buf(a.pref, N), bU = field_buf(a.u_in, N, DIM);
+    T f[Q], gg[Q], unused[Q];
+    pull<T, S, DIM, DEV, false, LOCAL>(bF, bF, x, nb, m, a.eps, bR, ph, f, unused);
+    pull_g<T, S, DIM, kGRef<S>, LOCAL>(bG, bU, (uint32_t)N, x, nb, m, a.eps, ph, gg);
     T p = a.pref[x];
     if constexpr (!LOCAL) {
       const T cf = (T)a.ctrl->c[a.slot];
@@ -1302,11 +1374,10 @@ __global__ __launch_bounds__(BS) void k_inject(InjectArgs<T, S, DIM> a, Phys<T> 
     T uu[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      T* up = a.u + (long long)d * N + x;
-      T un = (T(1) - C) * *up + C * a.uimp[d];
+      T un = (T(1) - C) * a.u_in[(long long)d * N + x] + C * a.uimp[d];
       if (solid) un = T(0);
       uu[d] = un;
-      *up = un;
+      a.u_out[(long long)d * N + x] = un;
     }
     if (core) {
       const T usq_t = divk(vsq<DIM>(uu), ph.two_cs2, ph.rcp_two_cs2);
@@ -1323,10 +1394,12 @@ __global__ __launch_bounds__(BS) void k_inject(InjectArgs<T, S, DIM> a, Phys<T> 
     const uint32_t vx = dvoff<Q, S>(x);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      T fv = f[i];
+      T fv = f[i], gv = gg[i];
       if constexpr (DEV) fv = fv - ph.template w<DIM>(i) * pn;
+      // local form: the slot is read back at this node, against its new u
+      if constexpr (kGRef<S>) gv = gv - ph.template w3<DIM>(i) * edot<DIM>(i, uu);
       dput<Q, T, S>(oF, i, vx, fv);
-      dput<Q, T, S>(oG, i, vx, gg[i]);
+      dput<Q, T, S>(oG, i, vx, gv);
     }
     a.phi_out[x] = pn;
     v = (double)pn;

@@ -1,8 +1,9 @@
 // Diagnostics reduced on the device (lbm/fe_measure.py:11-82 and the driver
 // metrics of scripts/run_thesis_sweep.py:185-206): extents of the liquid
 // (phi > 0.5 and not solid), volume (sum of phi with solid set to 0), C_max,
-// z_com (mean last-axis coordinate of the liquid nodes), NaN flag; and the
-// contact-line radius on the first fluid layer above each column's surface.
+// z_com (mean last-axis coordinate of the liquid nodes), NaN flag; the
+// contact-line radius on the first fluid layer above each column's surface;
+// and the base radius of the drop on its lowest liquid layer.
 // Each block writes one partial record; a one-block kernel combines the
 // records in a fixed order, so results do not depend on scheduling.
 #pragma once
@@ -113,7 +114,8 @@ __device__ __forceinline__ void fold_rec(MeasRec& a, const MeasRec& b) {
 
 template <int DIM, int BS>
 __global__ __launch_bounds__(BS) void k_measure_final(const MeasRec* __restrict__ parts,
-                                                      unsigned int n, double* out) {
+                                                      unsigned int n, double* out,
+                                                      int* zmin_out = nullptr) {
   __shared__ MeasRec s[BS];
   MeasRec acc;
   rec_empty(acc);
@@ -136,7 +138,41 @@ __global__ __launch_bounds__(BS) void k_measure_final(const MeasRec* __restrict_
     out[7] = (double)acc.cnt;
     out[8] = acc.vol;
     out[9] = (double)acc.nan;
+    // lowest liquid layer (last axis), -1 without liquid
+    if (zmin_out) *zmin_out = acc.cnt > 0 ? acc.mn[DIM - 1] : -1;
   }
+}
+
+// Base radius of the drop: one thread per column of the lowest liquid layer
+// *zmin (from k_measure_final); per-block max of r^2 over its liquid nodes.
+template <typename T, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_base(Grid g, Phys<T> ph, const T* __restrict__ phi,
+                                             const uint8_t* __restrict__ meta, const Ctrl* ctrl,
+                                             int slot, int scaled, double cx, double cy,
+                                             const int* __restrict__ zmin,
+                                             double* __restrict__ parts) {
+  __shared__ double s[BS];
+  const long long col = (long long)blockIdx.x * BS + threadIdx.x;
+  const long long ncol = (long long)g.nx * g.ny;
+  double r2 = -1.0;
+  const int z = *zmin;
+  if (DIM == 3 && col < ncol && z >= 0 && z < g.nz) {
+    const long long x = int_index<DIM>(g, (int)(col / g.ny), (int)(col % g.ny), z);
+    const uint8_t m = meta[x];
+    const T cf = scaled ? (T)ctrl->c[slot] : T(1);
+    const T p = phi_observed<T, DIM>(ph, phi, x, m, cf, scaled);
+    if (!(m & M_SOLID) && p > T(0.5)) {
+      const double dx = (double)(col / g.ny) - cx, dy = (double)(col % g.ny) - cy;
+      r2 = dx * dx + dy * dy;
+    }
+  }
+  s[threadIdx.x] = r2;
+  __syncthreads();
+  for (int w = BS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && s[threadIdx.x + w] > s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) parts[blockIdx.x] = s[0];
 }
 
 // Contact-line radius: one thread per column; per-block max of r^2.

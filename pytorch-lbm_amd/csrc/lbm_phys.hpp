@@ -22,6 +22,7 @@ template <typename T> struct Phys {
   T woc[3];           // w / cs2                 (lbm/fe_ac_solver.py:306,318)
   T lc[3];            // 2 * (w / cs2)           (lbm/fe_ac_solver.py:319)
   T lic[3];           // 2 * w                   (lbm/fe_ac_solver.py:293)
+  T w3c[3];           // T(3 w), the bf16 g reference's weight
   T cs2, two_cs2;
   T rcp_cs2, rcp_two_cs2, rcp_xi;   // RN(1/cs2), RN(1/(2 cs2)), RN(1/xi)
   T four_beta, kappa, rho_l, rho_g, rho_floor, drho, visc, xi, ac_scale;
@@ -55,6 +56,7 @@ template <typename T> struct Phys {
   T mu_drive_a;       // mu_drive * mobility
 
   template <int DIM> __host__ __device__ T w(int i) const { return wc[wclass<DIM>(i)]; }
+  template <int DIM> __host__ __device__ T w3(int i) const { return w3c[wclass<DIM>(i)]; }
 };
 
 }  // namespace lbm

@@ -88,6 +88,8 @@ _SIGS = {
     "lbm_measure_async": (ctypes.c_int, [_vp, _vp, _vp]),
     "lbm_contact_radius": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
                                           ctypes.POINTER(ctypes.c_double), _vp]),
+    "lbm_base_radius": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double,
+                                       ctypes.POINTER(ctypes.c_double), _vp]),
     "lbm_last_error": (ctypes.c_char_p, []),
     "lbm_graph_capture": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbm_check_division": (ctypes.c_int, [ctypes.c_float, ctypes.POINTER(ctypes.c_ulonglong),

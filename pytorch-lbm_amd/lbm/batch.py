@@ -273,6 +273,14 @@ class BatchEngine:
                                              self._stream()), "lbm_contact_radius")
         return out.value
 
+    def base_radius(self, cid):
+        spec = self.cases[cid]
+        out = ctypes.c_double()
+        nat.check(nat.lib.lbm_base_radius(self._solver(cid), spec.grid[0] / 2.0,
+                                          spec.grid[1] / 2.0, ctypes.byref(out),
+                                          self._stream()), "lbm_base_radius")
+        return out.value
+
 
 def droplet_center(spec: CaseSpec, solid):
     """The sweep driver's droplet placement (scripts/run_thesis_sweep.py:

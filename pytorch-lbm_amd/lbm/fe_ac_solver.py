@@ -427,6 +427,17 @@ class AllenCahnSolver:
                                              self._stream()), "lbm_contact_radius")
         return out.value
 
+    def base_radius(self, cx=None, cy=None):
+        """Max distance from (cx, cy) of liquid nodes on the lowest layer that
+        holds liquid (3D): the radius of the drop's base, also where it rides
+        on a gas cushion above the substrate."""
+        cx = self.shape[0] / 2.0 if cx is None else cx
+        cy = self.shape[1] / 2.0 if cy is None else cy
+        out = ctypes.c_double()
+        nat.check(nat.lib.lbm_base_radius(self._h, float(cx), float(cy), ctypes.byref(out),
+                                          self._stream()), "lbm_base_radius")
+        return out.value
+
     def inject_droplet(self, center, radius, xi, u_impact=None):
         """Add a droplet between steps (lbm/fe_ac_solver.py:645-732).
 

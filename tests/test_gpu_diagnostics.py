@@ -5,7 +5,7 @@ import pytest
 import torch
 
 from _cases import config_kwargs, load, solver_kwargs
-from oracle.ac_oracle import contact_radius, spread
+from oracle.ac_oracle import base_radius, contact_radius, spread
 
 pytestmark = pytest.mark.gpu
 
@@ -40,6 +40,7 @@ def test_measure_and_contact_radius(name):
         if phi.ndim == 3:
             cx, cy = phi.shape[0] / 2.0, phi.shape[1] / 2.0
             assert s.contact_radius(cx, cy) == contact_radius(phi, solid, cx, cy)
+            assert s.base_radius(cx, cy) == base_radius(phi, solid, cx, cy)
         # the non-blocking form writes the same record to device memory
         dev = s.measure_async()
         keys = ("Dx", "Dy", "Dz", "k", "volume", "C_max", "z_com", "n_liquid", "phi_sum", "nan")

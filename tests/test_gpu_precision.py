@@ -97,3 +97,62 @@ def test_16bit_pass_b_forms_agree(name, storage):
     for other in runs[1:]:
         for k in ("phi", "u", "f", "g"):
             np.testing.assert_array_equal(other[k], runs[0][k])
+
+
+# ---------------------------------------------------------------------------
+# The precision gate at full size: the 150x150x107 ridge impact (README
+# construction) over 2000 steps at theta in {90, 120, 162} degrees (no,
+# quadratic and linear wetting), every storage form against an fp64 run of the
+# same kernels: max spread factor and max base radius within 1%.
+# ---------------------------------------------------------------------------
+def _cfg3_run(theta, form, steps=2000, every=20):
+    from geometry.substrate import create_substrate_with_fraction
+    from lbm.fe_ac_solver import AllenCahnSolver
+    from lbm.fe_config import FEConfig
+    from lbm.fe_droplet import create_fe_droplet_with_impact
+    cfg = FEConfig(nx=150, ny=150, nz=107, rho_l=1.0, rho_g=1.0 / 828.0, sigma=0.01424,
+                   xi=4.0, tau_l=0.53, tau_g=0.53, theta_eq=theta, device="cuda")
+    dt = torch.float64 if form == "fp64" else torch.float32
+    storage = form if form in ("bf16", "fp16") else None
+    s = AllenCahnSolver(cfg, dtype=dt, geo_amplification=1.5, geometric_wetting=True,
+                        storage=storage)
+    solid, frac = create_substrate_with_fraction(150, 150, 107, "ridge", 1.0, 22.5)
+    s.set_solid(solid, solid_fraction=frac)
+    phi0, _, u0 = create_fe_droplet_with_impact(150, 150, 107, center=(75, 75, 80),
+                                                radius=22.5, xi=4.0, u_impact=(0, 0, -0.05))
+    s.init_fields(phi0, u0)
+    dmax = rmax = 0.0
+    for _ in range(steps // every):
+        s.step(every)
+        m = s.measure()
+        assert m["nan"] == 0
+        dmax = max(dmax, m["Dx"], m["Dy"])
+        rmax = max(rmax, s.base_radius(75.0, 75.0))
+    fluid = ~solid
+    out = (dmax / 45.0, rmax, s.phi.double().cpu().numpy()[fluid],
+           s.u.double().cpu().numpy()[:, fluid])
+    del s
+    torch.cuda.empty_cache()
+    return out
+
+
+@pytest.mark.parametrize("theta", [90.0, 120.0, 162.0])
+def test_cfg3_full_size_gate(theta):
+    need_gpu()
+    print(f"[gpu] cfg3 150x150x107 theta {theta:g}: fp64 reference", flush=True)
+    sf64, rb64, phi64, u64 = _cfg3_run(theta, "fp64")
+    worst = {}
+    for form in ("fp32", "bf16", "fp16"):
+        sf, rb, phi, u = _cfg3_run(theta, form)
+        e_sf = abs(sf / sf64 - 1.0)
+        e_rb = abs(rb / rb64 - 1.0) if rb64 > 0 else abs(rb)
+        e_phi = float(np.linalg.norm(phi - phi64) / np.linalg.norm(phi64))
+        e_u = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+        print(f"[gpu] cfg3 theta {theta:g} {form}: spread {sf:.4f} (fp64 {sf64:.4f}, "
+              f"{100 * e_sf:.2f}%), base radius {rb:.3f} (fp64 {rb64:.3f}, {100 * e_rb:.2f}%), "
+              f"phi L2 {e_phi:.2e}, u L2 {e_u:.2e}", flush=True)
+        worst[form] = (e_sf, e_rb)
+    assert rb64 > 5.0          # the base radius measures a real footprint
+    for form, (e_sf, e_rb) in worst.items():
+        assert e_sf <= 0.01, (form, e_sf)
+        assert e_rb <= 0.01, (form, e_rb)

@@ -140,3 +140,62 @@ def test_failed_case_is_isolated():
         np.testing.assert_array_equal(req.result["phi"].numpy(), ref[tag].result["phi"].numpy())
         for k in ("Dx", "Dy", "Dz", "volume", "contact_radius"):
             assert req.result[k] == ref[tag].result[k]
+
+
+def test_poisson_stream_at_real_sizes():
+    """A seeded Poisson stream (4 requests/s) of impact solves at the stated
+    grid sizes (nx = ny from 64 to 192, nz from 64 to 160; steps cut to
+    500-1500 so the test stays short), bf16 storage: requests are admitted by
+    HBM footprint against a budget that holds only a few of the large ones,
+    arrive and are admitted while others run, and every result equals the same
+    request run alone, bit for bit.  The reference serves a sweep one process
+    per case from a queue file (scripts/night_queue.py:28-50)."""
+    need_gpu()
+    print("[gpu] serve: Poisson stream at real sizes", flush=True)
+    from lbm.batch import BatchEngine, footprint
+    from lbm.serve import SweepServer, poisson_requests
+    reqs = poisson_requests(seed=11, rate=4.0, duration=3.0, steps=(500, 1500),
+                            grid_xy=(64, 192), grid_z=(64, 160))
+    assert len(reqs) >= 8
+    sizes = [r.spec.nodes for r in reqs]
+    assert max(sizes) >= 3_000_000 and min(sizes) <= 1_500_000
+    fps = sorted(footprint(r.spec, "bf16") for r in reqs)
+    budget = fps[-1] + fps[-2]                     # about two large cases
+    slots = 8
+    log = []
+    srv = SweepServer("bf16", budget_bytes=budget, max_active=slots, chunk=50)
+    held_by_memory = []
+    fits = srv._fits
+
+    def fits_logged(req):
+        ok = fits(req)
+        if not ok and len(srv.active) < slots:
+            held_by_memory.append(req.spec.tag)
+        return ok
+    srv._fits = fits_logged
+
+    def on_event(kind, req):
+        log.append((kind, req.spec.tag, len(srv.active), srv.engine.bytes_used()))
+    done = srv.run(reqs, keep_fields=True, on_event=on_event)
+    assert len(done) == len(reqs)
+    assert max(b for *_, b in log) <= budget
+    admits = [e for e in log if e[0] == "admit"]
+    # admitted while other cases were on the device
+    assert any(n >= 2 for _, _, n, _ in admits)
+    # memory, not slots, held some request back while slots were free
+    assert held_by_memory, "no request ever waited for device memory"
+    tts = np.array([r.t_done - r.t_arrival for r in done])
+    print(f"[gpu] serve: {len(done)} requests, p50 {np.percentile(tts, 50):.2f} s, "
+          f"p99 {np.percentile(tts, 99):.2f} s, {srv.lattice_updates / srv.wall / 1e6:.0f} MLUPS",
+          flush=True)
+    for req in done:
+        alone = BatchEngine("bf16")
+        cid = alone.add(req.spec)
+        alone.step(req.spec.steps)
+        np.testing.assert_array_equal(alone.field(cid, "phi").cpu().numpy(),
+                                      req.result["phi"].numpy())
+        m = alone.measure(cid)
+        for k in ("Dx", "Dy", "Dz", "volume"):
+            assert m[k] == req.result[k], (req.spec.tag, k)
+        del alone
+        torch.cuda.empty_cache()

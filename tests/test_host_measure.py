@@ -97,3 +97,31 @@ def test_monitor_film_and_gradient_angle():
     want = math.degrees(math.acos(b / math.hypot(a, b)))
     left, right = mon.compute_contact_angle(rho2, solid)
     assert abs(left - want) < 1e-9 and abs(right - want) < 1e-9
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_base_radius_brute_force(seed):
+    """oracle.base_radius against a plain loop over every node: lowest layer
+    holding liquid, then the farthest liquid node of that layer."""
+    import math
+    from oracle.ac_oracle import base_radius
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = 7, 6, 9
+    phi = rng.uniform(0.0, 1.0, (nx, ny, nz)) * (rng.uniform(size=(nx, ny, nz)) < 0.3)
+    solid = rng.uniform(size=(nx, ny, nz)) < 0.15
+    if seed == 3:
+        phi[:] = 0.2          # no liquid at all
+    cx, cy = 3.5, 2.0
+    zlow = None
+    for z in range(nz):
+        for x in range(nx):
+            for y in range(ny):
+                if phi[x, y, z] > 0.5 and not solid[x, y, z]:
+                    zlow = z if zlow is None else min(zlow, z)
+    want = 0.0
+    if zlow is not None:
+        for x in range(nx):
+            for y in range(ny):
+                if phi[x, y, zlow] > 0.5 and not solid[x, y, zlow]:
+                    want = max(want, math.sqrt((x - cx) ** 2 + (y - cy) ** 2))
+    assert base_radius(phi, solid, cx, cy) == want
