@@ -1,16 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark: MLUPS of the plain fp32 D3Q19 Allen-Cahn step on the
-150x150x107 ridge-impact case, its HBM roofline fraction, and a CPU baseline.
+"""Benchmark of the Allen-Cahn LBM step on one MI355X, printed as one JSON line.
 
-Workload (the README construction of the reference, fp32): FEConfig(nx=150,
-ny=150, nz=107, rho_l=1, rho_g=1/828, sigma=0.01424, xi=4, tau_l=tau_g=0.53,
-theta_eq=162), AllenCahnSolver(dtype=float32, geo_amplification=1.5,
-geometric_wetting=True) with boundary_relax=0.3 and tau_h=1.0; ridge R*=1,
-R_d=22.5 with solid fraction; droplet R=22.5 at (75,75,80), u=(0,0,-0.05).
-Inputs are resident in HBM before the timed region; a "step" is one
-solver.step() over all 2,407,500 nodes (solid included).
+value: MLUPS of the plain fp32 D3Q19 step on the 150x150x107 ridge impact,
+with `roofline` (the collide kernel against the 8 TB/s HBM peak), the step's
+% of the HBM roofline at 356 algorithmic B/LU, and beside it
+  sweep    the 48-case uneven impact sweep (1500 steps each) batched in bf16
+           storage against the same cases run one after another in the plain
+           fp32 form: solves/hour and the speed-up;
+  serve    a seeded Poisson stream (4 requests/s for 20 s) of impact solves
+           served with continuous batching: p50/p99 time to solution and the
+           aggregate MLUPS;
+  cpu_baseline            the step restated in torch on the CPU
+                          (oracle/torch_step.py) on every host core, on the
+                          same case;
+  cpu_baseline_reference  the reference's own step timed on the CPU in the
+                          build container (a committed fixture).
 
-Prints one JSON line (rank 0).  Only one GPU is supported.
+Workload of `value` (the README construction of the reference, fp32):
+FEConfig(nx=150, ny=150, nz=107, rho_l=1, rho_g=1/828, sigma=0.01424, xi=4,
+tau_l=tau_g=0.53, theta_eq=162), AllenCahnSolver(dtype=float32,
+geo_amplification=1.5, geometric_wetting=True) with boundary_relax=0.3 and
+tau_h=1.0; ridge R*=1, R_d=22.5 with solid fraction; droplet R=22.5 at
+(75,75,80), u=(0,0,-0.05).  Inputs are resident in HBM before the timed
+region; a "step" is one solver.step() over all 2,407,500 nodes (solid
+included).  Only one GPU is supported.
 """
 import argparse
 import json
@@ -59,26 +72,70 @@ def cfg2(storage=None, dtype="float32", theta=162.0):
     return s, (solid, frac, phi0, u0)
 
 
-def cpu_baseline(inputs, nsteps=2):
-    """The numpy oracle (a single-threaded port of the reference step) on the
-    same inputs: `nsteps` steps of the full 150x150x107 case."""
+def cpu_baseline(inputs, seconds=15.0, threads=None):
+    """The reference's CPU path, restated: oracle/torch_step.py (the step in
+    torch on the CPU, pinned against the numpy oracle) on the same inputs,
+    with torch.set_num_threads(os.cpu_count()), timed over whole steps of the
+    full 150x150x107 case for about `seconds` of CPU work."""
+    import torch
     from oracle.ac_oracle import OracleSolver, Params
+    from oracle.torch_step import TorchStep
     solid, frac, phi0, u0 = inputs
+    threads = threads or os.cpu_count() or 1
     p = Params(nx=150, ny=150, nz=107, rho_l=1.0, rho_g=1.0 / 828.0, sigma=0.01424, xi=4.0,
                tau_l=0.53, tau_g=0.53, theta_eq=162.0)
     o = OracleSolver(p, dtype=np.float32, geo_amplification=1.5, geometric_wetting=True)
     o.set_solid(solid, frac)
     o.init_fields(phi0, u0)
-    t0 = time.perf_counter()
-    for _ in range(nsteps):
-        o.step()
-    dt = time.perf_counter() - t0
+    prev = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    try:
+        t = TorchStep(o)
+        t0 = time.perf_counter()
+        t.step()                                    # first step: warm-up, and the pace
+        first = time.perf_counter() - t0
+        nsteps = max(2, min(40, int(seconds / max(first, 1e-3))))
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            t.step()
+        dt = time.perf_counter() - t0
+    finally:
+        torch.set_num_threads(prev)
     n = 150 * 150 * 107
-    return {"value": round(n * nsteps / dt / 1e6, 4), "unit": "MLUPS", "cores": 1,
+    return {"value": round(n * nsteps / dt / 1e6, 4), "unit": "MLUPS", "cores": threads,
             "kind": "port",
-            "sample": f"{nsteps} steps of the 150x150x107 ridge case, numpy oracle "
-                      f"(float32, single thread), {dt:.1f} s on "
-                      f"{platform.processor() or platform.machine()}"}
+            "sample": f"{nsteps} steps (after 1 untimed) of the 150x150x107 ridge case: "
+                      f"oracle/torch_step.py, torch {torch.__version__} on the CPU, float32, "
+                      f"{threads} threads, {dt:.1f} s on {cpu_model()}"}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def reference_cpu_timing():
+    """The reference's own AllenCahnSolver.step() timed on the CPU in the
+    build container (tests/golden/make_cfg2_reference.py writes the fixture):
+    reported beside the port's figure, labelled as measured elsewhere."""
+    path = os.path.join(ROOT, "tests", "golden", "reference_cpu_timing.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return {"value": round(rec["MLUPS"], 4), "unit": "MLUPS", "cores": rec["torch_threads"],
+            "kind": "reference",
+            "sample": f"{rec['steps_timed']} steps after {rec['warmup']} of the same workload, "
+                      f"the reference's own step (torch {rec['torch']}, CPU, float32, "
+                      f"{rec['torch_threads']} threads) on {rec['cpu_model'].strip(': ')}; "
+                      "measured in the build container, not on this host"}
 
 
 def hbm_traffic(storage):
@@ -168,10 +225,8 @@ def sweep_bench(steps=1500, every=50, storage="bf16", seed=2024, n=48):
     set_solid, init_fields, step loop), both with on-device diagnostics every
     `every` steps."""
     import torch
-    from geometry.substrate import create_substrate_with_fraction
-    from lbm.batch import BatchEngine, sweep_cases
+    from lbm.batch import BatchEngine, host_inputs, sweep_cases
     from lbm.fe_ac_solver import AllenCahnSolver
-    from lbm.fe_droplet import create_fe_droplet_with_impact
     cases = sweep_cases(seed, n, steps)
     lu = sum(c.nodes for c in cases) * steps
 
@@ -179,7 +234,10 @@ def sweep_bench(steps=1500, every=50, storage="bf16", seed=2024, n=48):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     eng = BatchEngine(storage)
-    ids = [eng.add(c) for c in cases]
+    # inputs built on the device (the library's generators; the sequential
+    # runs below take the numpy generators, which may differ in the last fp32
+    # bit of a few tanh values)
+    ids = [eng.add(c, inputs="device") for c in cases]
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t0
     spread_b = {cid: 0.0 for cid in ids}
@@ -221,15 +279,8 @@ def sweep_bench(steps=1500, every=50, storage="bf16", seed=2024, n=48):
         sol = AllenCahnSolver(cfg, dtype=torch.float32, geo_amplification=c.geo_amplification,
                               geometric_wetting=c.geometric_wetting,
                               boundary_relax=c.boundary_relax)
-        solid, frac = create_substrate_with_fraction(nx, ny, nz, c.substrate, c.R_star, c.R_drop)
+        solid, frac, phi0, u0 = host_inputs(c)
         sol.set_solid(solid, solid_fraction=frac)
-        has = solid.any(axis=2)
-        top = np.where(has, nz - 1 - np.argmax(solid[:, :, ::-1], axis=2), 0)
-        ridge_top = int(top.max()) if c.substrate == "concave" else int(top[nx // 2, ny // 2])
-        cz = min(ridge_top + 2.0 + c.R_drop, nz - c.R_drop - 2)
-        phi0, _, u0 = create_fe_droplet_with_impact(nx, ny, nz, center=(nx / 2.0, ny / 2.0, cz),
-                                                    radius=c.R_drop, xi=c.xi,
-                                                    u_impact=(0.0, 0.0, c.u_impact))
         sol.init_fields(phi0, u0)
         torch.cuda.synchronize()
         ts = time.perf_counter()
@@ -285,7 +336,8 @@ def serve_bench(duration=20.0, rate=4.0, seed=5, storage="bf16", slots=32, chunk
     reqs = poisson_requests(seed, rate, duration)
     free, total = torch.cuda.mem_get_info()
     budget = int(0.9 * free)
-    srv = SweepServer(storage, budget_bytes=budget, max_active=slots, chunk=chunk)
+    srv = SweepServer(storage, budget_bytes=budget, max_active=slots, chunk=chunk,
+                      inputs="device")
     peak = {"active": 0, "bytes": 0}
 
     def on_event(kind, req):
@@ -359,8 +411,12 @@ def main():
                     help="time eager launches instead of replays of a captured step graph")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape (256-thread blocks, no occupancy cap)")
-    ap.add_argument("--cpu-steps", type=int, default=6,
-                    help="steps of the numpy oracle timed as the CPU baseline (~1.2 s each)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU work of the torch-CPU baseline sample (whole steps)")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the 48-case batched sweep in the default line")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the Poisson serving run in the default line")
     ap.add_argument("--mode", default="plain", choices=["plain", "precision", "sweep", "serve"],
                     help="plain: the roofline/MLUPS line (default); precision: storage forms "
                          "against fp64 on the same case; sweep: 48 uneven cases batched vs "
@@ -468,9 +524,35 @@ def main():
         "mass_drift": abs(mass - mass0) / mass0,
         "diag": {k: meas[k] for k in ("Dx", "Dy", "Dz", "k", "z_com", "nan")},
     }
+    del s
+    torch.cuda.empty_cache()
+    if not args.no_sweep:
+        print("[bench] sweep ...", file=sys.stderr, flush=True)
+        sw = sweep_bench()
+        rec["sweep"] = {k: sw[k] for k in ("value", "unit", "speedup_vs_sequential",
+                                           "stepping_speedup")}
+        rec["sweep"].update(
+            workload="48 uneven D3Q19 impact cases (tests/golden/cfg4_cases.json) x 1500 "
+                     "steps, batched bf16 storage vs sequential plain fp32",
+            batched_MLUPS=sw["batched"]["MLUPS"], batched_wall_s=sw["batched"]["wall_s"],
+            sequential_wall_s=sw["sequential_fp32"]["wall_s"],
+            sequential_solves_per_hour=sw["sequential_fp32"]["solves_per_hour"],
+            spread_max_rel_diff=sw["spread_factor_max_rel_diff_batched_vs_sequential"])
+    if not args.no_serve:
+        print("[bench] serve ...", file=sys.stderr, flush=True)
+        sv = serve_bench(rate=args.rate)
+        rec["serve"] = {k: sv[k] for k in ("p50_time_to_solution_s", "p99_time_to_solution_s",
+                                           "aggregate_MLUPS", "offered_MLUPS", "requests",
+                                           "rate_per_s", "p99_queue_wait_s", "peak_active",
+                                           "storage", "any_nan")}
+        rec["serve"]["workload"] = ("seeded Poisson stream, 4 requests/s for 20 s, grids "
+                                    "64^3 .. 256x256x160, 500-3000 steps, continuous batching")
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        rec["cpu_baseline"] = cpu_baseline(inputs, args.cpu_steps)
+        rec["cpu_baseline"] = cpu_baseline(inputs, args.cpu_seconds)
+        ref = reference_cpu_timing()
+        if ref is not None:
+            rec["cpu_baseline_reference"] = ref
     print(json.dumps(rec), flush=True)
 
 
