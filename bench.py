@@ -411,6 +411,9 @@ def main():
                     help="time eager launches instead of replays of a captured step graph")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape (256-thread blocks, no occupancy cap)")
+    ap.add_argument("--launch", default=None,
+                    help="fixed launch shape instead of the autotuner: block A, block B, "
+                         "cap A, cap B, form of pass B (e.g. 256,256,0,4,2)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU work of the torch-CPU baseline sample (whole steps)")
     ap.add_argument("--no-sweep", action="store_true",
@@ -457,7 +460,12 @@ def main():
     sixteen = args.storage in ("bf16", "fp16")
     b_step = B_STEP_16 if sixteen else B_STEP
     b_coll = B_COLLIDE_16 if sixteen else B_COLLIDE
-    if not args.no_autotune:
+    if args.launch:
+        ba, bb, ca, cb, form = (int(v) for v in args.launch.split(","))
+        s.set_split(form)
+        s.set_occupancy(ca, cb)
+        s.set_block_sizes(ba, bb)
+    elif not args.no_autotune:
         s.autotune(5)               # part of the warmup: 162 real steps
     s.step(max(args.warmup, 1))
     # the timed steps run as replays of a captured graph of g steps (even,

@@ -42,14 +42,19 @@ DRIVER_SRC = os.path.join(os.path.dirname(HERE), "tests", "c_abi", "abi_driver.c
 DRIVER = os.path.join(os.path.dirname(DRIVER_SRC), "abi_driver")
 
 
-def build(verbose=True, jobs=None):
-    os.makedirs(OUT_DIR, exist_ok=True)
-    os.makedirs(OBJ_DIR, exist_ok=True)
+def build(verbose=True, jobs=None, out=None, extra=(), obj_dir=None):
+    """Compile every unit (in parallel) and link `out` (default: the
+    in-tree library).  `extra`: additional compiler flags, e.g. -D tuning
+    macros for a variant build written elsewhere."""
+    out = out or OUT
+    obj_dir = obj_dir or OBJ_DIR
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    os.makedirs(obj_dir, exist_ok=True)
     jobs = jobs or int(os.environ.get("MAX_JOBS", "0")) or min(5, os.cpu_count() or 1)
     cmds = []
     for name, src, defs in UNITS:
-        obj = os.path.join(OBJ_DIR, name + ".o")
-        cmds.append((obj, [HIPCC] + CFLAGS + defs + ["-c", "-o", obj, src]))
+        obj = os.path.join(obj_dir, name + ".o")
+        cmds.append((obj, [HIPCC] + CFLAGS + list(extra) + defs + ["-c", "-o", obj, src]))
     pending = list(cmds)
     running = []
     while pending or running:
@@ -63,13 +68,14 @@ def build(verbose=True, jobs=None):
             for _, other in running:
                 other.wait()
             raise subprocess.CalledProcessError(proc.returncode, cmd)
-    link = [HIPCC] + FLAGS + ["-o", OUT + ".tmp"] + [obj for obj, _ in cmds]
+    link = [HIPCC] + FLAGS + ["-o", out + ".tmp"] + [obj for obj, _ in cmds]
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)
-    os.replace(OUT + ".tmp", OUT)
-    build_driver(verbose)
-    return OUT
+    os.replace(out + ".tmp", out)
+    if out == OUT:
+        build_driver(verbose)
+    return out
 
 
 def build_driver(verbose=True):

@@ -845,6 +845,10 @@ __device__ __forceinline__ void correct_grad(const Phys<T>& ph, T* gr, const flo
 // ---------------------------------------------------------------------------
 // Pass B: everything from S2 to S8 at one node, then node-local stores.
 // ---------------------------------------------------------------------------
+#ifndef LBM_EARLY_F
+#define LBM_EARLY_F 0
+#endif
+
 template <typename T, typename S, int DIM> struct CollideArgs {
   Grid g;
   const S* __restrict__ Fin;
@@ -907,6 +911,12 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   // 142.1 -> 140.6 us/step for bf16; no gain for 4-byte storage)
   if constexpr (DEV) __builtin_amdgcn_s_setprio(1);
   pull_g<T, S, DIM, kGRef<S>, LOCAL, CP_G>(bGi, bU, N32, x, nb, m, a.eps, ph, gg);
+  T f[Q];
+#if LBM_EARLY_F
+  // the f pulls leave together with the g pulls: their latency hides behind
+  // the g collision (f and g then live together: more registers)
+  pull<T, S, DIM, DEV, false, LOCAL, CP_FLD>(bFi, bFi, x, nb, m, a.eps, bPp, ph, f, unused);
+#endif
   if constexpr (DEV) __builtin_amdgcn_s_setprio(0);
   T pprev = T(0);
   if (((m & M_TOP) && !LOCAL) || a.phiu_valid) pprev = a.phi_prev[x];
@@ -1084,13 +1094,16 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   // the neighbour indices are rebuilt here rather than kept live from the
   // top (18 registers across the whole node); the empty asm hides x's
   // identity so the compiler cannot merge the two computations
+#if LBM_EARLY_F
+  const Nbrs<DIM>& nb2 = nb;
+#else
   uint32_t x2 = x;
   asm volatile("" : "+v"(x2));
   const Nbrs<DIM> nb2 = neighbours<DIM>(g, x2, decompose<DIM>(g, x2));
-  T f[Q];
   if constexpr (DEV) __builtin_amdgcn_s_setprio(1);
   pull<T, S, DIM, DEV, false, LOCAL, CP_FLD>(bFi, bFi, x, nb2, m, a.eps, bPp, ph, f, unused);
   if constexpr (DEV) __builtin_amdgcn_s_setprio(0);
+#endif
   if constexpr (!LOCAL)
     scale_f<T, DIM>(ph, f, (T)a.ctrl->c[a.slot_prev], m, pprev, nullptr,
                     nsw_phi<T>(ph, m, a.phi_prev, x));

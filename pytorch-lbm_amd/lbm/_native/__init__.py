@@ -13,6 +13,9 @@ import torch  # noqa: F401  (loads the HIP runtime the library links against)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblbm_amd.so")
+# an alternative build of the same library (e.g. one compiled with other
+# tuning macros) can be selected for measurements
+LIB_PATH = os.environ.get("LBM_AMD_LIB", LIB_PATH)
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
