@@ -199,6 +199,18 @@ size_t lbm_field_bytes(const lbm_solver* s, int which);
 int lbm_grad_lap(lbm_solver* s, const void* phi_dev, void* grad_dev, void* lap_dev,
                  void* stream);
 
+/* lbm_grad_lap without allocations, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same inputs and
+ * results as lbm_grad_lap; asynchronous on `stream` (no synchronisation).
+ * Its scratch -- phi reordered into the library's node order -- is a static
+ * device array of 32 MiB held by every GPU the library runs on until the
+ * process ends: LBM_EINVAL when N * sizeof(run dtype) exceeds it (8,388,608
+ * nodes in fp32 and the 16-bit forms, 4,194,304 in fp64).  Calls that overlap
+ * on one GPU share that array and race; calls on one stream are ordered. */
+int lbm_grad_lap_noalloc(lbm_solver* s, const void* phi_dev, void* grad_dev, void* lap_dev,
+                         void* stream);
+
 /* Mass bookkeeping: M0 (phi_mass_init) and the fp64 sum of the last
  * start-of-step phi.  Synchronises `stream`. */
 int lbm_get_mass(lbm_solver* s, double* m0, double* m_last, void* stream);

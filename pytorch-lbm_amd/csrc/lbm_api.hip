@@ -10,6 +10,8 @@
 // synchronises, so step sequences are graph-capturable.
 #include "lbm_impl.hpp"
 
+#include <mutex>
+
 thread_local std::string g_err;
 
 // ---------------------------------------------------------------------------
@@ -597,6 +599,67 @@ int lbm_grad_lap(lbm_solver* s, const void* phi_dev, void* grad_dev, void* lap_d
                  void* stream) {
   if (!s || !phi_dev || !grad_dev || !lap_dev) return fail(LBM_EINVAL, "null argument");
 #define CALL(T, D) gradlap_impl<T, D>(s, phi_dev, grad_dev, lap_dev, (hipStream_t)stream)
+  return DISPATCH(s, CALL);
+#undef CALL
+}
+
+// ---------------------------------------------------------------------------
+// The stencil operator without allocations (lbm_grad_lap_noalloc).  Its one
+// scratch buffer -- phi reordered into the internal node order -- is a
+// file-scope device array: every GPU holds its own copy for the life of the
+// process, so the entry point calls no allocator.  Host code never uses the
+// array's name as an address: its device address is looked up once per
+// device with hipGetSymbolAddress and kept in g_scratch_dev.  Calls that
+// overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// Device address of this device's scratch array (looked up on first use).
+static int scratch_on_current_device(void** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(LBM_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(LBM_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return LBM_OK;
+}
+
+template <typename T, int DIM>
+static int gradlap_noalloc_impl(lbm_solver* s, const void* phi, void* grad, void* lap,
+                                hipStream_t st) {
+  if ((size_t)s->N * sizeof(T) > kScratchBytes)
+    return fail(LBM_EINVAL, "grid too large for lbm_grad_lap_noalloc: phi must fit its " +
+                                std::to_string(kScratchBytes) + "-byte scratch");
+  void* tmp = nullptr;
+  int rc = scratch_on_current_device(&tmp);
+  if (rc) return rc;
+  const Phys<T> ph = make_phys<T, DIM>(s);
+  rc = to_internal(s, phi, tmp, sizeof(T), 1, st);
+  if (rc) return rc;
+  k_gradlap<T, DIM, BS><<<s->nblocks, BS, 0, st>>>(s->g, ph, (const T*)tmp,
+                                                   s->geom ? s->meta : nullptr, s->wmask,
+                                                   s->normal, (T*)grad, (T*)lap);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_grad_lap_noalloc(lbm_solver* s, const void* phi_dev, void* grad_dev, void* lap_dev,
+                         void* stream) {
+  if (!s || !phi_dev || !grad_dev || !lap_dev) return fail(LBM_EINVAL, "null argument");
+#define CALL(T, D) gradlap_noalloc_impl<T, D>(s, phi_dev, grad_dev, lap_dev, (hipStream_t)stream)
   return DISPATCH(s, CALL);
 #undef CALL
 }

@@ -82,6 +82,7 @@ _SIGS = {
     "lbm_get_field": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "lbm_field_bytes": (ctypes.c_size_t, [_vp, ctypes.c_int]),
     "lbm_grad_lap": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "lbm_grad_lap_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "lbm_get_mass": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_double), _vp]),
     "lbm_set_mass_target": (ctypes.c_int, [_vp, ctypes.c_double]),
