@@ -1,6 +1,8 @@
 """Output formats: the VTK writer against a value-by-value rendering of the
 reference's file layout (io_utils/vtk_writer.py:51-86), 2D and 3D, with and
 without a solid mask."""
+import os
+
 import numpy as np
 
 from io_utils.vtk_writer import write_vtk
@@ -48,7 +50,33 @@ def test_vtk_matches_reference_layout(tmp_path):
 
 
 def test_snapshot_keys(tmp_path):
-    from lbm.snapshots import snapshot_arrays  # noqa: F401  (import check)
-    import importlib
-    m = importlib.import_module("lbm.snapshots")
-    assert hasattr(m, "save_snapshot")
+    """save_snapshot writes the reference's sweep snapshot: file name
+    <label>_step<step:05d>.npz and the keys, shapes and dtypes of
+    scripts/run_thesis_sweep.py:225-241 (phi_3d, u_3d, solid_3d, the two
+    mid-plane phi slices, the solid slice, step, nx, ny, nz)."""
+    import torch
+    from lbm.snapshots import save_snapshot
+
+    class Fields:                       # the attributes save_snapshot reads
+        pass
+    nx, ny, nz = 6, 5, 4
+    rng = np.random.default_rng(3)
+    f = Fields()
+    phi = rng.random((nx, ny, nz)).astype(np.float32)
+    u = rng.standard_normal((3, nx, ny, nz)).astype(np.float32)
+    solid = rng.random((nx, ny, nz)) < 0.2
+    f.phi, f.u, f.solid = torch.from_numpy(phi), torch.from_numpy(u), torch.from_numpy(solid)
+    path = save_snapshot(f, str(tmp_path / "snaps"), "ridge_th162", 250)
+    assert os.path.basename(path) == "ridge_th162_step00250.npz"
+    z = np.load(path)
+    assert sorted(z.files) == sorted(["phi_3d", "u_3d", "solid_3d", "xz_slice", "yz_slice",
+                                      "xz_solid", "step", "nx", "ny", "nz"])
+    np.testing.assert_array_equal(z["phi_3d"], phi)
+    np.testing.assert_array_equal(z["u_3d"], u)
+    np.testing.assert_array_equal(z["solid_3d"], solid)
+    assert z["phi_3d"].dtype == np.float32 and z["u_3d"].dtype == np.float32
+    assert z["solid_3d"].dtype == np.bool_
+    np.testing.assert_array_equal(z["xz_slice"], phi[:, ny // 2, :])
+    np.testing.assert_array_equal(z["yz_slice"], phi[nx // 2, :, :])
+    np.testing.assert_array_equal(z["xz_solid"], solid[:, ny // 2, :])
+    assert (int(z["step"]), int(z["nx"]), int(z["ny"]), int(z["nz"])) == (250, nx, ny, nz)
