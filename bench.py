@@ -242,7 +242,8 @@ def sweep_bench(steps=1500, every=50, storage="bf16", seed=2024, n=48):
     t_gen = time.perf_counter() - t0
     spread_b = {cid: 0.0 for cid in ids}
     eng.step(1)
-    tune = eng.autotune(3)
+    eng.autotune(3)
+    tune = eng.launch()
     done = eng.status(ids[0])[0]
     g = every if every % 2 == 0 else every + 1
     reps = (steps - done) // g
@@ -311,7 +312,9 @@ def sweep_bench(steps=1500, every=50, storage="bf16", seed=2024, n=48):
         "speedup_vs_sequential": round(t_s / t_b, 3),
         "batched": {"wall_s": round(t_b, 3), "generation_s": round(t_gen, 3),
                     "stepping_s": round(t_b_step, 3), "MLUPS": round(lu / t_b / 1e6, 1),
-                    "storage": storage, "graph_steps": g, "block_sizes": list(tune)},
+                    "storage": storage, "graph_steps": g,
+                    "launch": dict(zip(("block_phi", "block_collide", "cap_phi", "cap_collide",
+                                        "split"), tune))},
         "sequential_fp32": {"wall_s": round(t_s, 3), "stepping_s": round(t_steps, 3),
                             "MLUPS": round(lu / t_s / 1e6, 1),
                             "solves_per_hour": round(n / t_s * 3600.0, 1),
@@ -416,6 +419,8 @@ def main():
                          "cap A, cap B, form of pass B (e.g. 256,256,0,4,2)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU work of the torch-CPU baseline sample (whole steps)")
+    ap.add_argument("--sweep-storage", default="fp16", choices=["fp32", "bf16", "fp16"],
+                    help="distribution storage of the batched sweep (serving stays bf16)")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the 48-case batched sweep in the default line")
     ap.add_argument("--no-serve", action="store_true",
@@ -440,7 +445,7 @@ def main():
         print(json.dumps(precision_report()), flush=True)
         return
     if args.mode == "sweep":
-        print(json.dumps(sweep_bench()), flush=True)
+        print(json.dumps(sweep_bench(storage=args.sweep_storage)), flush=True)
         return
     if args.mode == "serve":
         print(json.dumps(serve_bench(rate=args.rate)), flush=True)
@@ -536,12 +541,13 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_sweep:
         print("[bench] sweep ...", file=sys.stderr, flush=True)
-        sw = sweep_bench()
+        sw = sweep_bench(storage=args.sweep_storage)
         rec["sweep"] = {k: sw[k] for k in ("value", "unit", "speedup_vs_sequential",
                                            "stepping_speedup")}
         rec["sweep"].update(
             workload="48 uneven D3Q19 impact cases (tests/golden/cfg4_cases.json) x 1500 "
-                     "steps, batched bf16 storage vs sequential plain fp32",
+                     f"steps, batched {args.sweep_storage} storage vs sequential plain fp32",
+            storage=args.sweep_storage, launch=sw["batched"]["launch"],
             batched_MLUPS=sw["batched"]["MLUPS"], batched_wall_s=sw["batched"]["wall_s"],
             sequential_wall_s=sw["sequential_fp32"]["wall_s"],
             sequential_solves_per_hour=sw["sequential_fp32"]["solves_per_hour"],

@@ -358,18 +358,32 @@ int lbm_batch_status(lbm_batch* b, int case_id, long long* steps_done, long long
 size_t lbm_batch_bytes_used(const lbm_batch* b);
 
 /* Advance every unfinished case by up to nsteps steps (cases stop at their
- * target): 3 launches per step for the whole batch, no host synchronisation. */
+ * target): 3 or 4 launches per step for the whole batch, no host
+ * synchronisation. */
 int lbm_batch_step(lbm_batch* b, int nsteps, void* stream);
 
 /* Block sizes of the batch kernels (128/256/512). */
 int lbm_batch_set_block_sizes(lbm_batch* b, int bs_phi, int bs_collide);
 
-/* Pass B of the batch as plain-fluid / other chunk-list launches (1, default)
- * or one launch over all nodes (0); results are identical. */
+/* Form of pass B for the whole batch, as lbm_set_split for one solver: 1
+ * (default) plain-fluid / other chunk-list launches, 2 one launch over every
+ * case's single-launch list (per XCD range plain chunks first, wall chunks
+ * last), 0 one launch over all nodes; results are identical.  LBM_EINVAL for
+ * other values.  Drops a captured batch graph. */
 int lbm_batch_set_split(lbm_batch* b, int split);
 
-/* Time the 9 block-size pairs over nsteps_each steps each (the batch advances
- * by 9*(nsteps_each+1) steps) and keep the fastest.  Synchronises. */
+/* Occupancy caps of the batch's pass A and pass B (waves per SIMD, 1..8; 0:
+ * none), as lbm_set_occupancy.  Drops a captured batch graph. */
+int lbm_batch_set_occupancy(lbm_batch* b, int cap_phi, int cap_collide);
+
+/* Current batch launch shape: out5 = {block size A, block size B, cap A,
+ * cap B, form of pass B}. */
+int lbm_batch_get_launch(const lbm_batch* b, int* out5);
+
+/* Autotuner of the batch launch shape: times the 9 (form of pass B, pass-B
+ * occupancy cap) pairs at 256-thread blocks, then the other 8 block-size
+ * pairs with the winner, over nsteps_each steps each (the batch advances by
+ * 17*(nsteps_each+1) steps), and keeps the fastest.  Synchronises. */
 int lbm_batch_autotune(lbm_batch* b, int nsteps_each, void* stream, int* bs_phi,
                        int* bs_collide);
 

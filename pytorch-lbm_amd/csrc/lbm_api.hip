@@ -945,7 +945,7 @@ int lbm_batch_destroy(lbm_batch* b) {
   batch_drop_graph(b);
   for (lbm_solver* s : b->cs)
     if (s) destroy_impl(s);
-  void* ptrs[] = {b->dcases, b->mapA, b->mapB, b->mapP, b->mapW, b->gen};
+  void* ptrs[] = {b->dcases, b->mapA, b->mapB, b->mapP, b->mapW, b->mapM, b->gen};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete b;
@@ -1157,9 +1157,26 @@ int lbm_batch_status(lbm_batch* b, int id, long long* done, long long* target) {
 
 size_t lbm_batch_bytes_used(const lbm_batch* b) { return b ? b->used : 0; }
 
+int lbm_batch_set_occupancy(lbm_batch* b, int cap_phi, int cap_collide) {
+  if (!b) return fail(LBM_EINVAL, "null batch");
+  if (cap_phi < 0 || cap_phi > 8 || cap_collide < 0 || cap_collide > 8)
+    return fail(LBM_EINVAL, "occupancy cap must be 0 (none) or 1..8 waves per SIMD");
+  b->capA = cap_phi;
+  b->capB = cap_collide;
+  return lbm_batch_set_block_sizes(b, b->bsA, b->bsB);
+}
+
+int lbm_batch_get_launch(const lbm_batch* b, int* out5) {
+  if (!b || !out5) return fail(LBM_EINVAL, "null argument");
+  out5[0] = b->bsA; out5[1] = b->bsB; out5[2] = b->capA; out5[3] = b->capB;
+  out5[4] = b->split;
+  return LBM_OK;
+}
+
 int lbm_batch_set_split(lbm_batch* b, int split) {
   if (!b) return fail(LBM_EINVAL, "null batch");
-  b->split = split != 0;
+  if (split < 0 || split > 2) return fail(LBM_EINVAL, "split must be 0, 1 or 2");
+  b->split = split;
   b->dirty = true;
   batch_drop_graph(b);
   return LBM_OK;
@@ -1171,6 +1188,8 @@ int lbm_batch_set_block_sizes(lbm_batch* b, int bsA, int bsB) {
   if (!ok(bsA) || !ok(bsB)) return fail(LBM_EINVAL, "block size must be 128, 256 or 512");
   b->bsA = bsA;
   b->bsB = bsB;
+  b->ldsA = lds_for_cap(bsA, b->capA, 0);
+  b->ldsB = lds_for_cap(bsB, b->capB, 1024);   // finalize_mass's 128 doubles
   b->dirty = true;
   batch_drop_graph(b);
   return LBM_OK;
@@ -1245,32 +1264,55 @@ int lbm_batch_autotune(lbm_batch* b, int nsteps_each, void* stream, int* bsA, in
   if (!b) return fail(LBM_EINVAL, "null batch");
   if (nsteps_each < 1) nsteps_each = 1;
   hipStream_t st = (hipStream_t)stream;
-  const int cand[3] = {128, 256, 512};
-  float best = 1e30f;
-  int bA = 256, bB = 256;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
+  // shape: {block A, block B, cap A, cap B, form of pass B}
+  int best[5] = {b->bsA, b->bsB, b->capA, b->capB, b->split};
+  float best_ms = 1e30f;
+  auto trial = [&](const int* c) -> int {
+    b->split = c[4];
+    b->capA = c[2];
+    b->capB = c[3];
+    int rc = lbm_batch_set_block_sizes(b, c[0], c[1]);
+    if (!rc) rc = lbm_batch_step(b, 1, stream);   // upload + first launch of this shape
+    if (rc) return rc;
+    (void)hipEventRecord(e0, st);
+    rc = lbm_batch_step(b, nsteps_each, stream);
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (!rc && ms < best_ms) {
+      best_ms = ms;
+      for (int k = 0; k < 5; ++k) best[k] = c[k];
+    }
+    return rc;
+  };
   int rc = LBM_OK;
-  for (int i = 0; i < 3 && rc == LBM_OK; ++i)
-    for (int j = 0; j < 3 && rc == LBM_OK; ++j) {
-      rc = lbm_batch_set_block_sizes(b, cand[i], cand[j]);
-      if (rc) break;
-      rc = lbm_batch_step(b, 1, stream);   // upload + first launch of this shape
-      if (rc) break;
-      (void)hipEventRecord(e0, st);
-      rc = lbm_batch_step(b, nsteps_each, stream);
-      (void)hipEventRecord(e1, st);
-      (void)hipEventSynchronize(e1);
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) { best = ms; bA = cand[i]; bB = cand[j]; }
+  // stage 1: form of pass B and its occupancy cap at 256-thread blocks;
+  // stage 2: the block sizes of both passes with the stage-1 winner
+  const int caps[3] = {0, 5, 4}, bsz[3] = {128, 256, 512};
+  for (int f = 0; f < 3 && !rc; ++f)
+    for (int k = 0; k < 3 && !rc; ++k) {
+      const int c[5] = {256, 256, 0, caps[k], f};
+      rc = trial(c);
+    }
+  const int form = best[4], capb = best[3];
+  for (int i = 0; i < 3 && !rc; ++i)
+    for (int k = 0; k < 3 && !rc; ++k) {
+      if (bsz[i] == 256 && bsz[k] == 256) continue;
+      const int c[5] = {bsz[i], bsz[k], 0, capb, form};
+      rc = trial(c);
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  rc = lbm_batch_set_block_sizes(b, bA, bB);
-  if (bsA) *bsA = bA;
-  if (bsB) *bsB = bB;
+  b->split = best[4];
+  b->capA = best[2];
+  b->capB = best[3];
+  rc = lbm_batch_set_block_sizes(b, best[0], best[1]);
+  if (bsA) *bsA = best[0];
+  if (bsB) *bsB = best[1];
   return rc;
 }

@@ -34,6 +34,8 @@ template <typename T, typename S> struct BatchCase {
   const uint32_t* chunks;      // pass-B chunk lists: plain chunks, then the rest
   unsigned int n_plain, n_wall;
   unsigned int nbP8, nbW8;     // blocks of the two chunk lists (XCD multiples)
+  const uint32_t* mixed;       // single-launch list for the batch's pass-B block size
+  unsigned int n_mixed, nbM8;  // its entries; its blocks rounded up to an XCD multiple
   long long k, k_end;          // steps done, step target
   int local, phiu_valid;
   int pad[2];
@@ -161,6 +163,57 @@ __global__ __launch_bounds__(BS) void k_collide_batch_chunks(
   a.phiu_valid = cs.phiu_valid;
   if (cs.local) collide_node<T, S, DIM, true, FLUID>(a, cs.ph, x);
   else collide_node<T, S, DIM, false, FLUID>(a, cs.ph, x);
+}
+
+// Pass B of every case in one launch over its single-launch chunk list (each
+// XCD's range of the case's blocks holds its plain chunks first and its wall
+// chunks last, as in k_collide_mixed; a case's blocks start at a multiple of
+// the XCD count, so block i of the case runs on XCD i % 8 as there).  A case
+// in the post-stream form (its first step) runs the general code throughout.
+template <typename T, typename S, int DIM, int BS>
+__global__ __launch_bounds__(BS) void k_collide_batch_mixed(
+    const BatchCase<T, S>* __restrict__ cases, BlockMap map) {
+  const uint32_t ci = map.cs[blockIdx.x];
+  const BatchCase<T, S>& cs = cases[ci];
+  if (cs.k >= cs.k_end) return;
+  const unsigned int i = blockIdx.x - map.base[blockIdx.x];
+  const int pslot = (int)(cs.k & 1), nslot = (int)((cs.k + 1) & 1);
+  const long long N = cs.g.N;
+  if (i == 0)
+    finalize_mass<BS>(cs.partials, (unsigned int)((N + 63) / 64), cs.ctrl, nslot);
+  const unsigned int nb = cs.n_mixed / (BS / 64);
+  if (i >= nb) return;
+  const unsigned int w = xcd_block(i, nb) * (BS / 64) + (threadIdx.x >> 6);
+  const uint32_t c = __builtin_amdgcn_readfirstlane(cs.mixed[w]);
+  if (c == 0xFFFFFFFFu) return;
+  const uint32_t x = ((c & 0x7FFFFFFFu) << 6) + (threadIdx.x & 63u);
+  if (x >= N) return;
+  CollideArgs<T, S, DIM> a;
+  a.g = cs.g;
+  a.Fin = cs.F[pslot];
+  a.Gin = cs.G[pslot];
+  a.Fout = cs.F[nslot];
+  a.Gout = cs.G[nslot];
+  a.u_in = cs.u[pslot];
+  a.u_out = cs.u[nslot];
+  a.phi_cur = cs.phi[nslot];
+  a.phi_prev = cs.phi[pslot];
+  a.meta = cs.meta;
+  a.wmask = cs.wmask;
+  a.normal = cs.normal;
+  a.eps = cs.eps;
+  a.ampf = cs.ampf;
+  a.ctrl = cs.ctrl;
+  a.ctrl_w = cs.ctrl;
+  a.partials = cs.partials;
+  a.nparts = 0;
+  a.slot_prev = pslot;
+  a.slot_new = nslot;
+  a.local = cs.local;
+  a.phiu_valid = cs.phiu_valid;
+  if (cs.local) collide_node<T, S, DIM, true, false>(a, cs.ph, x);
+  else if (c >> 31) collide_node<T, S, DIM, false, false>(a, cs.ph, x);
+  else collide_node<T, S, DIM, false, true>(a, cs.ph, x);
 }
 
 // After pass B: every case that stepped advances its counter.

@@ -623,9 +623,14 @@ struct lbm_batch {
   uint32_t* mapB = nullptr;
   uint32_t* mapP = nullptr;           // pass B over plain-fluid chunks
   uint32_t* mapW = nullptr;           // pass B over the other chunks
-  size_t mapA_cap = 0, mapB_cap = 0, mapP_cap = 0, mapW_cap = 0;
-  unsigned int nA = 0, nB = 0, nP = 0, nW = 0;   // grid sizes
-  bool split = true;                  // pass B as the two chunk-list launches
+  uint32_t* mapM = nullptr;           // pass B over the single-launch lists
+  size_t mapA_cap = 0, mapB_cap = 0, mapP_cap = 0, mapW_cap = 0, mapM_cap = 0;
+  unsigned int nA = 0, nB = 0, nP = 0, nW = 0, nM = 0;   // grid sizes
+  // form of pass B: 0 one launch over all nodes, 1 the two chunk-list
+  // launches, 2 one launch over the single-launch lists
+  int split = 1;
+  int capA = 0, capB = 0;             // occupancy caps (waves per SIMD, 0: none)
+  unsigned int ldsA = 0, ldsB = 0;    // dynamic LDS per block enforcing them
   int ncase = 0;                      // descriptor count (== cs.size())
   // scratch for device-side case generation
   void* gen = nullptr;
@@ -653,7 +658,7 @@ int batch_upload(lbm_batch* b, hipStream_t st) {
   batch_drop_graph(b);
   const int n = (int)b->cs.size();
   std::vector<BatchCase<T, S>> h((size_t)std::max(n, 1));
-  std::vector<uint32_t> mA, mB, mP, mW;
+  std::vector<uint32_t> mA, mB, mP, mW, mM;
   for (int c = 0; c < n; ++c) {
     BatchCase<T, S>& d = h[c];
     std::memset(&d, 0, sizeof(d));
@@ -683,6 +688,10 @@ int batch_upload(lbm_batch* b, hipStream_t st) {
     d.n_plain = s->n_plain;
     d.n_wall = s->n_wall;
     const unsigned int wpb = (unsigned int)(b->bsB / 64);
+    const int mk = b->bsB == 128 ? 0 : (b->bsB == 256 ? 1 : 2);
+    d.mixed = s->mixed[mk];
+    d.n_mixed = s->n_mixed[mk];
+    d.nbM8 = (d.n_mixed / wpb + kXCD - 1) / kXCD * kXCD;
     d.nbP8 = ((s->n_plain + wpb - 1) / wpb + kXCD - 1) / kXCD * kXCD;
     d.nbW8 = ((s->n_wall + wpb - 1) / wpb + kXCD - 1) / kXCD * kXCD;
     d.k = s->k;
@@ -693,6 +702,8 @@ int batch_upload(lbm_batch* b, hipStream_t st) {
     const uint32_t baseA = (uint32_t)(mA.size() / 2), baseB = (uint32_t)(mB.size() / 2);
     for (unsigned int i = 0; i < d.nbA8; ++i) { mA.push_back((uint32_t)c); mA.push_back(baseA); }
     for (unsigned int i = 0; i < d.nbB8; ++i) { mB.push_back((uint32_t)c); mB.push_back(baseB); }
+    const uint32_t baseM = (uint32_t)(mM.size() / 2);
+    for (unsigned int i = 0; i < d.nbM8; ++i) { mM.push_back((uint32_t)c); mM.push_back(baseM); }
     const uint32_t baseP = (uint32_t)(mP.size() / 2), baseW = (uint32_t)(mW.size() / 2);
     if (s->n_plain)
       for (unsigned int i = 0; i < d.nbP8; ++i) { mP.push_back((uint32_t)c); mP.push_back(baseP); }
@@ -721,6 +732,7 @@ int batch_upload(lbm_batch* b, hipStream_t st) {
   HIPCHK(upload_map(mB, &b->mapB, &b->mapB_cap, &b->nB));
   HIPCHK(upload_map(mP, &b->mapP, &b->mapP_cap, &b->nP));
   HIPCHK(upload_map(mW, &b->mapW, &b->mapW_cap, &b->nW));
+  HIPCHK(upload_map(mM, &b->mapM, &b->mapM_cap, &b->nM));
   const size_t bytes = h.size() * sizeof(BatchCase<T, S>);
   if (bytes > b->dcases_cap) {
     if (b->dcases) { HIPCHK(hipStreamSynchronize(st)); (void)hipFree(b->dcases); }
@@ -739,28 +751,37 @@ void batch_launch(lbm_batch* b, hipStream_t st) {
   const BatchCase<T, S>* dc = (const BatchCase<T, S>*)b->dcases;
   const unsigned int nA = b->nA, nB = b->nB;
   BlockMap ma{b->mapA, b->mapA + nA}, mb{b->mapB, b->mapB + nB};
+  const unsigned int lA = b->ldsA, lB = b->ldsB;
   if (nA) {
-#define LA(B) k_phi_batch<T, S, DIM, B><<<nA, B, 0, st>>>(dc, ma)
+#define LA(B) k_phi_batch<T, S, DIM, B><<<nA, B, lA, st>>>(dc, ma)
     LBM_BS_SWITCH(b->bsA, LA)
 #undef LA
   }
-  if (!b->split) {
+  if (b->split == 0) {
     if (nB) {
-#define LB(B) k_collide_batch<T, S, DIM, B><<<nB, B, 0, st>>>(dc, mb)
+#define LB(B) k_collide_batch<T, S, DIM, B><<<nB, B, lB, st>>>(dc, mb)
       LBM_BS_SWITCH(b->bsB, LB)
 #undef LB
+    }
+  } else if (b->split == 2) {
+    const unsigned int nM = b->nM;
+    BlockMap mm{b->mapM, b->mapM + nM};
+    if (nM) {
+#define LM(B) k_collide_batch_mixed<T, S, DIM, B><<<nM, B, lB, st>>>(dc, mm)
+      LBM_BS_SWITCH(b->bsB, LM)
+#undef LM
     }
   } else {
     // plain-fluid chunks of every case, then the other chunks
     const unsigned int nP = b->nP, nW = b->nW;
     BlockMap mp{b->mapP, b->mapP + nP}, mw{b->mapW, b->mapW + nW};
     if (nP) {
-#define LP(B) k_collide_batch_chunks<T, S, DIM, B, true><<<nP, B, 0, st>>>(dc, mp)
+#define LP(B) k_collide_batch_chunks<T, S, DIM, B, true><<<nP, B, lB, st>>>(dc, mp)
       LBM_BS_SWITCH(b->bsB, LP)
 #undef LP
     }
     if (nW) {
-#define LW(B) k_collide_batch_chunks<T, S, DIM, B, false><<<nW, B, 0, st>>>(dc, mw)
+#define LW(B) k_collide_batch_chunks<T, S, DIM, B, false><<<nW, B, lB, st>>>(dc, mw)
       LBM_BS_SWITCH(b->bsB, LW)
 #undef LW
     }

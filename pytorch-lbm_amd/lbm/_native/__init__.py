@@ -123,6 +123,8 @@ _SIGS = {
     "lbm_batch_step": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "lbm_batch_set_block_sizes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "lbm_batch_set_split": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbm_batch_set_occupancy": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "lbm_batch_get_launch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "lbm_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int)]),
     "lbm_batch_graph_capture": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),

@@ -189,9 +189,21 @@ class BatchEngine:
         return a.value, b.value
 
     def set_split(self, split):
-        """Pass B over plain-fluid / wall chunk lists (True, default) or all
-        nodes in one launch."""
-        nat.check(nat.lib.lbm_batch_set_split(self._h, 1 if split else 0), "lbm_batch_set_split")
+        """Form of pass B: 1 (or True, default) plain-fluid / wall chunk-list
+        launches, 2 one launch over every case's single-launch list, 0 (or
+        False) one launch over all nodes; results are identical."""
+        nat.check(nat.lib.lbm_batch_set_split(self._h, int(split)), "lbm_batch_set_split")
+
+    def set_occupancy(self, cap_phi, cap_collide):
+        """Cap resident waves per SIMD of the two batch passes (0: no cap)."""
+        nat.check(nat.lib.lbm_batch_set_occupancy(self._h, int(cap_phi), int(cap_collide)),
+                  "lbm_batch_set_occupancy")
+
+    def launch(self):
+        """(block size A, block size B, cap A, cap B, form of pass B)."""
+        out = (ctypes.c_int * 5)()
+        nat.check(nat.lib.lbm_batch_get_launch(self._h, out), "lbm_batch_get_launch")
+        return tuple(out)
 
     def capture_graph(self, nsteps):
         with torch.cuda.device(self.device):

@@ -45,13 +45,40 @@ def test_same_alone_and_in_the_mix(storage):
         assert mix.status(cid) == (s.steps, s.steps)
     for k, (cid, s) in enumerate(zip(ids, sp)):
         alone = BatchEngine(storage)
-        alone.set_split(k % 2)               # one launch or the chunk lists: same result
+        alone.set_split(k % 3)               # every form of pass B: same result
+        alone.set_occupancy(0, (0, 4, 5)[k % 3])
         a = alone.add(s)
         alone.step(s.steps)
         for name in ("phi", "u", "f"):
             np.testing.assert_array_equal(alone.field(a, name).cpu().numpy(),
                                           mix.field(cid, name).cpu().numpy())
         assert alone.measure(a) == mix.measure(cid)
+
+
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+def test_single_launch_form_and_autotune(storage):
+    """The single-launch pass-B form of the batch (and whatever shape the
+    autotuner picks) gives the default form's results bit for bit, cases in
+    their first (post-stream form) step included."""
+    need_gpu()
+    from lbm.batch import BatchEngine
+    sp = specs()
+    ref = BatchEngine(storage)
+    ir = [ref.add(s) for s in sp]
+    ref.step(40)
+    mx = BatchEngine(storage)
+    im = [mx.add(s) for s in sp]
+    mx.set_split(2)
+    mx.step(3)
+    mx.autotune(1)                       # 17 shapes x 2 steps
+    launch = mx.launch()
+    assert launch[4] in (0, 1, 2) and launch[1] in (128, 256, 512)
+    mx.step(40)
+    for a, b in zip(ir, im):
+        assert mx.status(b) == ref.status(a)
+        for name in ("phi", "u", "f", "g"):
+            np.testing.assert_array_equal(ref.field(a, name).cpu().numpy(),
+                                          mx.field(b, name).cpu().numpy())
 
 
 def test_batch_graph_replay_matches_eager():
