@@ -377,6 +377,9 @@ __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, con
 // fp16 (10-bit mantissa) stores g as is.
 // ---------------------------------------------------------------------------
 template <typename S> constexpr bool kGRef = std::is_same<S, __hip_bfloat16>::value;
+#ifndef LBM_GREF_EARLY
+#define LBM_GREF_EARLY 0
+#endif
 
 // r_i(y) from the velocity field bU (DIM arrays of N32 values).
 template <typename T, int DIM>
@@ -408,6 +411,13 @@ __device__ __forceinline__ void pull_g(const Buf& G, const Buf& bU, uint32_t N32
     }
 #pragma unroll
     for (int i = 0; i < Q; ++i) g[i] = dload<Q, T, S, CP>(G, i, dvoff<Q, S>(nb.i[OPP<DIM>(i)]));
+#if LBM_GREF_EARLY
+    // the sources' velocities are loaded together with the g slots (one
+    // memory latency instead of two in series)
+    T r[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) r[i] = gref<T, DIM>(ph, i, bU, N32, nb.i[OPP<DIM>(i)]);
+#endif
     __builtin_amdgcn_sched_barrier(0);
     if (m & M_SOLID) {
 #pragma unroll
@@ -415,8 +425,13 @@ __device__ __forceinline__ void pull_g(const Buf& G, const Buf& bU, uint32_t N32
         g[i] = dload<Q, T, S>(G, OPP<DIM>(i), vx) + gref<T, DIM>(ph, OPP<DIM>(i), bU, N32, x);
       return;
     }
+#if LBM_GREF_EARLY
+#pragma unroll
+    for (int i = 0; i < Q; ++i) g[i] = g[i] + r[i];
+#else
 #pragma unroll
     for (int i = 0; i < Q; ++i) g[i] = g[i] + gref<T, DIM>(ph, i, bU, N32, nb.i[OPP<DIM>(i)]);
+#endif
     if (m & M_PARTIAL) {
       const T eps = eps_arr[x];
       const T a = T(1) - eps;
