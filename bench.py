@@ -471,7 +471,9 @@ def main():
         s.set_occupancy(ca, cb)
         s.set_block_sizes(ba, bb)
     elif not args.no_autotune:
-        s.autotune(5)               # part of the warmup: 162 real steps
+        # 27 launch shapes x 11 real steps (297 steps): its timings decide
+        # the shape, and they run before the warm-up steps
+        s.autotune(10)
     s.step(max(args.warmup, 1))
     # the timed steps run as replays of a captured graph of g steps (even,
     # dividing the step count; an odd count runs its last step eagerly),
