@@ -134,7 +134,7 @@ def reference_cpu_timing():
             "kind": "reference",
             "sample": f"{rec['steps_timed']} steps after {rec['warmup']} of the same workload, "
                       f"the reference's own step (torch {rec['torch']}, CPU, float32, "
-                      f"{rec['torch_threads']} threads) on {rec['cpu_model'].strip(': ')}; "
+                      f"{rec['torch_threads']} threads) on {rec['cpu_model'].split(':', 1)[-1].strip()}; "
                       "measured in the build container, not on this host"}
 
 

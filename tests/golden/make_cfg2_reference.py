@@ -104,7 +104,7 @@ def main():
                    "torch_threads": nthreads, "torch": torch.__version__,
                    "cpu": platform.processor() or "see lscpu",
                    "cpu_model": open("/proc/cpuinfo").read().split(
-                       "model name")[1].split("\n")[0].strip(": "),
+                       "model name")[1].split("\n")[0].split(":", 1)[-1].strip(),
                    "logical_cpus": os.cpu_count()}
             with open(os.path.join(HERE, "reference_cpu_timing.json"), "w") as f:
                 json.dump(rec, f, indent=1)
