@@ -417,6 +417,8 @@ def main():
     ap.add_argument("--launch", default=None,
                     help="fixed launch shape instead of the autotuner: block A, block B, "
                          "cap A, cap B, form of pass B (e.g. 256,256,0,4,2)")
+    ap.add_argument("--fused-tile", default=None,
+                    help="ty,tx,zlen of the fused step (form 3 of --launch; 0 = automatic)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU work of the torch-CPU baseline sample (whole steps)")
     ap.add_argument("--sweep-storage", default="fp16", choices=["fp32", "bf16", "fp16"],
@@ -470,6 +472,8 @@ def main():
         s.set_split(form)
         s.set_occupancy(ca, cb)
         s.set_block_sizes(ba, bb)
+        if args.fused_tile:
+            s.set_fused_tile(*(int(v) for v in args.fused_tile.split(",")))
     elif not args.no_autotune:
         # 27 launch shapes x 11 real steps (297 steps): its timings decide
         # the shape, and they run before the warm-up steps

@@ -169,10 +169,25 @@ int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide);
  * storage).  2: one launch over both lists, arranged so that each XCD runs
  * its plain chunks first and its wall chunks last (the wall chunks fill the
  * launch's tail; the kernel carries the general code's registers; the
- * default for fp32).  0: one launch over all nodes.  Results are identical in
- * every form; the autotuner picks one.  LBM_EINVAL for other values.  No
- * counterpart in the reference. */
+ * default for fp32).  0: one launch over all nodes.  3: the fused step --
+ * pass A and pass B in one launch, each workgroup marching a tile of columns
+ * through a segment of planes with the phi stencil in an LDS ring (3D with
+ * fp32 or fp64 storage and no grad-mu form; elsewhere form 2 runs; the first
+ * step after init_fields takes form 1).  Results are identical in every form
+ * (the fp64 mass sum of a fused step adds the same values in another order);
+ * the autotuner picks one.  LBM_EINVAL for other values.  No counterpart in
+ * the reference. */
 int lbm_set_split(lbm_solver* s, int split);
+
+/* Tile of the fused step: ty x tx columns (y, x) per workgroup, zlen planes
+ * per segment; 0 chooses automatically (the default).  LBM_EINVAL when
+ * ty * tx exceeds the pass-B block size or (ty + 2) * (tx + 2) twice it.
+ * Invalidates a captured graph.  No counterpart in the reference. */
+int lbm_set_fused_tile(lbm_solver* s, int ty, int tx, int zlen);
+
+/* Fused-step plan in use: out6 = {ty, tx, tiles along y, tiles along x,
+ * zlen, segments}; zeros when the fused form is not selected. */
+int lbm_get_fused_tile(const lbm_solver* s, int* out6);
 
 /* Current launch shape: out5 = {block size A, block size B, cap A, cap B,
  * form of pass B (lbm_set_split)}. */

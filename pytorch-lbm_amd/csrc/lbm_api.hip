@@ -149,7 +149,7 @@ static int destroy_impl(lbm_solver* s) {
                   s->eps, s->ampf, s->meta, s->bnd, s->dist[0], s->dist[1], s->solid,
                   s->wmask, s->normal, s->surf, s->partials, s->mparts, s->dout, s->ctrl,
                   s->chunks, s->chunk_flag, s->keep, s->nsw, s->mu,
-                  s->mixed[0], s->mixed[1], s->mixed[2]};
+                  s->mixed[0], s->mixed[1], s->mixed[2], s->fpart};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete s;
@@ -473,6 +473,51 @@ int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide) {
   s->ldsA = lds_for_cap(s->bsA, s->capA, 0);
   s->ldsB = lds_for_cap(s->bsB, s->capB, 1024);   // finalize_mass's 128 doubles
   drop_graph(s);
+  s->f_blocks = 0;
+  if (s->split == 3 && fused_ok(s)) {
+    int pl[6];
+    if (!fused_plan(s->g, s->bsB, s->freq_ty, s->freq_tx, s->freq_zlen, pl))
+      return fail(LBM_EINVAL, "fused tile does not fit the pass-B block size "
+                              "(ty * tx <= block and (ty + 2) * (tx + 2) <= 2 * block)");
+    const unsigned int nb = (unsigned int)(pl[2] * pl[3] * pl[5]);
+    if (nb > s->fpart_cap) {
+      if (s->fpart) (void)hipFree(s->fpart);
+      s->fpart = nullptr;
+      s->fpart_cap = 0;
+      HIPCHK(hipMalloc(&s->fpart, (size_t)nb * sizeof(double)));
+      s->fpart_cap = nb;
+    }
+    s->f_ty = pl[0]; s->f_tx = pl[1]; s->f_nty = pl[2]; s->f_ntx = pl[3];
+    s->f_zlen = pl[4]; s->f_nseg = pl[5];
+    s->f_blocks = nb;
+    // four ring planes; the occupancy cap pads the same dynamic allocation
+    const unsigned int ring = 4u * (unsigned int)((pl[0] + 2) * (pl[1] + 2)) * (unsigned int)s->esz;
+    const unsigned int tot = lds_for_cap(s->bsB, s->capB, 0);
+    s->ldsF = ring > tot ? ring : tot;
+  }
+  return LBM_OK;
+}
+
+int lbm_set_fused_tile(lbm_solver* s, int ty, int tx, int zlen) {
+  if (!s) return fail(LBM_EINVAL, "null solver");
+  REFUSE_BATCHED(s);
+  if (ty < 0 || tx < 0 || zlen < 0) return fail(LBM_EINVAL, "tile sizes must be >= 0 (0: automatic)");
+  const int o[3] = {s->freq_ty, s->freq_tx, s->freq_zlen};
+  s->freq_ty = ty;
+  s->freq_tx = tx;
+  s->freq_zlen = zlen;
+  const int rc = lbm_set_block_sizes(s, s->bsA, s->bsB);
+  if (rc) {
+    s->freq_ty = o[0]; s->freq_tx = o[1]; s->freq_zlen = o[2];
+    (void)lbm_set_block_sizes(s, s->bsA, s->bsB);
+  }
+  return rc;
+}
+
+int lbm_get_fused_tile(const lbm_solver* s, int* out6) {
+  if (!s || !out6) return fail(LBM_EINVAL, "null argument");
+  out6[0] = s->f_ty; out6[1] = s->f_tx; out6[2] = s->f_nty; out6[3] = s->f_ntx;
+  out6[4] = s->f_zlen; out6[5] = s->f_nseg;
   return LBM_OK;
 }
 
@@ -489,9 +534,15 @@ int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide) {
 int lbm_set_split(lbm_solver* s, int split) {
   if (!s) return fail(LBM_EINVAL, "null solver");
   REFUSE_BATCHED(s);
-  if (split < 0 || split > 2) return fail(LBM_EINVAL, "split must be 0, 1 or 2");
+  if (split < 0 || split > 3) return fail(LBM_EINVAL, "split must be 0, 1, 2 or 3");
+  const int old = s->split;
   s->split = split;
-  return lbm_set_block_sizes(s, s->bsA, s->bsB);
+  const int rc = lbm_set_block_sizes(s, s->bsA, s->bsB);
+  if (rc) {
+    s->split = old;
+    (void)lbm_set_block_sizes(s, s->bsA, s->bsB);
+  }
+  return rc;
 }
 
 int lbm_get_launch(const lbm_solver* s, int* out5) {

@@ -93,8 +93,17 @@ struct lbm_solver {
   uint32_t* mixed[3] = {nullptr, nullptr, nullptr};
   unsigned int n_mixed[3] = {0, 0, 0};
   // pass B: 0 one launch over all nodes, 1 the two chunk-list launches,
-  // 2 one launch over the per-XCD mixed list
+  // 2 one launch over the per-XCD mixed list; 3 the fused step (one launch
+  // for both passes; 3D with distributions stored in the compute type and no
+  // grad-mu form, else form 2; the first step after init_fields always takes
+  // the two chunk-list launches)
   int split = 1;
+  // fused step: requested tile (0: automatic) and the plan in use
+  int freq_ty = 0, freq_tx = 0, freq_zlen = 0;
+  int f_ty = 0, f_tx = 0, f_nty = 0, f_ntx = 0, f_zlen = 0, f_nseg = 0;
+  unsigned int f_blocks = 0, ldsF = 0;
+  double* fpart = nullptr;    // one fp64 sum of phi per fused block
+  unsigned int fpart_cap = 0;
   int capA = 0, capB = 0;     // occupancy caps, waves per SIMD (0: none)
   unsigned int ldsA = 0, ldsB = 0;   // dynamic LDS per block enforcing the caps
   hipGraphExec_t gexec = nullptr;
@@ -257,6 +266,57 @@ inline unsigned int lds_for_cap(int bs, int cap, unsigned int stat) {
   return tot > stat ? tot - stat : 0;
 }
 
+// Can this solver take the fused step?
+inline bool fused_ok(const lbm_solver* s) {
+  return s->dim == 3 && s->esz == s->dsz && s->gmu == GMU_NONE;
+}
+
+// Tile and z segments of the fused step for blocks of bs threads.  ty, tx:
+// tile extent along y and x (0: choose), zlen: planes per segment (0:
+// choose).  The automatic tile minimises the collision lanes launched
+// (ragged edge tiles leave lanes idle) plus the ring work (a ring node costs
+// about 0.14 of a collision: pass A's share of the step's VALU work), with a
+// 30% penalty on rows that are neither whole grid rows nor 64 nodes long
+// (a wave's accesses then split into more, partly used cache lines: on the
+// 150x150x107 grid 50x5 tiles ran 338 us, whole-row 150x3 tiles 255 us); the
+// automatic segments give about four blocks per CU (256 CUs).  Returns false
+// for a tile that does not fit the block (ty * tx > bs or a ring plane of
+// more than 2 bs values).
+inline bool fused_plan(const Grid& g, int bs, int ty, int tx, int zlen, int* out6) {
+  auto ceil_div = [](int a, int b) { return (a + b - 1) / b; };
+  if (ty <= 0 || tx <= 0) {
+    double best = 1e300;
+    int bty = 0, btx = 0;
+    for (int cy = 1; cy <= (g.ny < bs ? g.ny : bs); ++cy) {
+      int cx = bs / cy;
+      if (cx > g.nx) cx = g.nx;
+      if (cx < 1 || (cy + 2) * (cx + 2) > 2 * bs) continue;
+      const double tiles = (double)ceil_div(g.ny, cy) * ceil_div(g.nx, cx);
+      double cost = tiles * (bs + 0.14 * (cy + 2) * (cx + 2));
+      if (cy < 64 && cy < g.ny) cost *= 1.3;
+      if (cost < best) { best = cost; bty = cy; btx = cx; }
+    }
+    if (!bty) return false;
+    ty = bty;
+    tx = btx;
+  }
+  if (ty > g.ny) ty = g.ny;
+  if (tx > g.nx) tx = g.nx;
+  if (ty * tx > bs || (ty + 2) * (tx + 2) > 2 * bs) return false;
+  const int nty = ceil_div(g.ny, ty), ntx = ceil_div(g.nx, tx);
+  if (zlen <= 0) {
+    const int target = 16 * 256 / (bs / 64);
+    int nseg = (int)((double)target / (nty * ntx) + 0.5);
+    if (nseg < 1) nseg = 1;
+    if (nseg > g.nz) nseg = g.nz;
+    zlen = ceil_div(g.nz, nseg);
+  }
+  if (zlen > g.nz) zlen = g.nz;
+  out6[0] = ty; out6[1] = tx; out6[2] = nty; out6[3] = ntx; out6[4] = zlen;
+  out6[5] = ceil_div(g.nz, zlen);
+  return true;
+}
+
 // A captured graph holds the buffer pointers, chunk lists and amplification
 // field of the moment of capture: every call that changes any of them drops it.
 inline void drop_graph(lbm_solver* s) {
@@ -353,6 +413,8 @@ int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev = nullpt
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 0], st));
     const int src = s->cur, dst = s->cur ^ 1;
     const int pslot = (int)(s->k & 1), nslot = (int)((s->k + 1) & 1);
+    const bool fused = s->split == 3 && !s->local && s->f_blocks > 0 && fused_ok(s);
+    if (!fused) {
 #define LAUNCH_A(B)                                                                        \
   if (s->local)                                                                             \
     k_phi<T, S, DIM, B, true><<<s->nbA, B, s->ldsA, st>>>(                                  \
@@ -365,6 +427,7 @@ int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev = nullpt
     LBM_BS_SWITCH(s->bsA, LAUNCH_A)
 #undef LAUNCH_A
     if (ev) HIPCHK(hipEventRecord(ev[3 * it + 1], st));
+    }
     CollideArgs<T, S, DIM> a;
     a.g = s->g;
     a.Fin = (const S*)s->Fb[src];
@@ -389,7 +452,25 @@ int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev = nullpt
     a.local = s->local ? 1 : 0;
     a.phiu_valid = s->phiu_valid ? 1 : 0;
     a.mu = (const T*)s->mu;
-    if (s->gmu != GMU_NONE) {
+    if (fused) {
+      if constexpr (DIM == 3 && std::is_same<T, S>::value) {
+        FusedTile ft;
+        ft.ty = s->f_ty;
+        ft.tx = s->f_tx;
+        ft.nty = s->f_nty;
+        ft.ntx = s->f_ntx;
+        ft.zlen = s->f_zlen;
+        ft.sy = s->f_ty + 2;
+        ft.box = (s->f_ty + 2) * (s->f_tx + 2);
+        ft.bpart = s->fpart;
+#define LAUNCH_F(B) \
+  k_fused<T, B><<<s->f_blocks, B, s->ldsF, st>>>(a, ph, ft, (T*)s->phi[nslot])
+        LBM_BS_SWITCH(s->bsB, LAUNCH_F)
+#undef LAUNCH_F
+        if (ev) HIPCHK(hipEventRecord(ev[3 * it + 1], st));
+        k_fused_mass<256><<<1, 256, 0, st>>>(s->fpart, s->f_blocks, s->ctrl, nslot);
+      }
+    } else if (s->gmu != GMU_NONE) {
       // grad mu needs mu at every neighbour: one more pass, then pass B in
       // its grad-mu form (one launch, 256 threads)
       k_mu<T, DIM, 256><<<nblk(s->N), 256, 0, st>>>(s->g, ph, (const T*)s->phi[nslot], s->meta,

@@ -717,19 +717,44 @@ __device__ __forceinline__ void finalize_mass(const double* __restrict__ partial
 // lbm/fe_wetting.py:418-446).  init_lap selects the stand-alone Laplacian of
 // init_fields (lbm/fe_ac_solver.py:282-294).
 // ---------------------------------------------------------------------------
-template <typename T, int DIM>
-__device__ __forceinline__ void stencil(const Phys<T>& ph, const Grid& g,
-                                        const T* __restrict__ phi, const Nbrs<DIM>& nb, T p,
-                                        uint32_t wm, const float* n32, T* grad, T* lap,
-                                        bool init_lap) {
+// Sources of the start-of-step phi for the stencil: the phi array of every
+// node, or (fused step) an LDS ring of phi planes around the block's tile.
+template <typename T> struct PhiGlobal {
+  const T* __restrict__ p;
+  Buf b;
+  __device__ __forceinline__ PhiGlobal(const T* ptr, long long N) : p(ptr), b(field_buf(ptr, N)) {}
+  __device__ __forceinline__ T self(uint32_t x) const { return at(p, x); }
+  template <int DIM> __device__ __forceinline__ T nbr(const Nbrs<DIM>& nb, int i) const {
+    return bget<T, T>(b, 0, 0, nb.i[i]);
+  }
+};
+// lm, l0, lp: the node's position in the ring planes z-1, z, z+1; sy: the
+// ring's row stride (one step along x; one step along y is 1).  3D only.
+template <typename T> struct PhiRing {
+  const T* lm;
+  const T* l0;
+  const T* lp;
+  int sy;
+  __device__ __forceinline__ T self(uint32_t) const { return l0[0]; }
+  template <int DIM> __device__ __forceinline__ T nbr(const Nbrs<DIM>&, int i) const {
+    static_assert(DIM == 3, "the LDS phi ring is 3D only");
+    const int ez = E<3>(i, 2);
+    const T* pl = ez > 0 ? lp : (ez < 0 ? lm : l0);
+    return pl[E<3>(i, 0) * sy + E<3>(i, 1)];
+  }
+};
+
+template <typename T, int DIM, typename PS>
+__device__ __forceinline__ void stencil_src(const Phys<T>& ph, const PS& ps, const Nbrs<DIM>& nb,
+                                            T p, uint32_t wm, const float* n32, T* grad, T* lap,
+                                            bool init_lap) {
   constexpr int Q = Lat<DIM>::Q;
-  const Buf bphi = field_buf(phi, g.N);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) grad[d] = T(0);
   T l = T(0);
 #pragma unroll
   for (int i = 1; i < Q; ++i) {
-    T psi = bget<T, T>(bphi, 0, 0, nb.i[i]);
+    T psi = ps.template nbr<DIM>(nb, i);
     if (wm & (1u << i)) {
       // n . e_i accumulated in fp32 from zero (lbm/fe_wetting.py:237-239)
       float nde = 0.0f;
@@ -759,6 +784,14 @@ __device__ __forceinline__ void stencil(const Phys<T>& ph, const Grid& g,
     }
   }
   *lap = l;
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ void stencil(const Phys<T>& ph, const Grid& g,
+                                        const T* __restrict__ phi, const Nbrs<DIM>& nb, T p,
+                                        uint32_t wm, const float* n32, T* grad, T* lap,
+                                        bool init_lap) {
+  stencil_src<T, DIM>(ph, PhiGlobal<T>(phi, g.N), nb, p, wm, n32, grad, lap, init_lap);
 }
 
 template <typename T>
@@ -893,9 +926,10 @@ template <typename T, typename S, int DIM> struct CollideArgs {
 // FLUID: the caller guarantees a plain fluid node (no solid, partial, wall or
 // top flag), so every wall path compiles away.  GMU: the interface source
 // uses grad mu (Cahn-Hilliard or the mu drive), from the mu of every node.
-template <typename T, typename S, int DIM, bool LOCAL, bool FLUID = false, bool GMU = false>
-__device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
-                                             const Phys<T>& ph, uint32_t x) {
+// PS: source of the start-of-step phi (PhiGlobal or PhiRing).
+template <typename T, typename S, int DIM, bool LOCAL, bool FLUID, bool GMU, typename PS>
+__device__ __forceinline__ void collide_node_src(const CollideArgs<T, S, DIM>& a,
+                                                 const Phys<T>& ph, uint32_t x, const PS& ps) {
   constexpr int Q = Lat<DIM>::Q;
   constexpr bool DEV = !std::is_same<T, S>::value;
   const Grid& g = a.g;
@@ -925,7 +959,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   T gg[Q], unused[Q];
   const uint32_t N32 = (uint32_t)N;
   const Buf bFi = dist_buf<DIM>(a.Fin, N), bGi = dist_buf<DIM>(a.Gin, N);
-  const Buf bPp = field_buf(a.phi_prev, N), bPc = field_buf(a.phi_cur, N);
+  const Buf bPp = field_buf(a.phi_prev, N);
   const Buf bU = field_buf(a.u_in, N, DIM), bUo = field_buf(a.u_out, N, DIM);
   // 16-bit storage (VALU-bound): a wave issuing its pulls runs at raised
   // priority, so its loads leave early while other waves compute (measured
@@ -943,7 +977,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
   if (((m & M_TOP) && !LOCAL) || a.phiu_valid) pprev = a.phi_prev[x];
 
   // S1/S2: phi, rho, stencil, mu, correction
-  const T p = at(a.phi_cur, x);
+  const T p = ps.self(x);
   const T rho = p * ph.rho_l + (T(1) - p) * ph.rho_g;
   const T rho_eff = rho < ph.rho_floor ? ph.rho_floor : rho;
   float n32[DIM];
@@ -957,7 +991,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
     for (int d = 0; d < DIM; ++d) n32[d] = 0.0f;
   }
   T gr[DIM], lap;
-  stencil<T, DIM>(ph, g, a.phi_cur, nb, p, wm, n32, gr, &lap, false);
+  stencil_src<T, DIM>(ph, ps, nb, p, wm, n32, gr, &lap, false);
   const T mu = chem_pot<T>(ph, p, lap, bnd && ph.wet != WET_NONE);
   T gmu[DIM];
   if constexpr (GMU) grad_mu<T, DIM>(ph, a.mu, nb, mu, wm, n32, gmu);
@@ -1142,9 +1176,15 @@ __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
     const T fe = feq_of<T, DIM>(ph, i, p, ec[i], usq_n);
     const T fs = (f[i] - ph.om_f * (f[i] - fe)) + fsrc[i];
     T fst = fs;
-    if constexpr (DEV) fst = fs - ph.template w<DIM>(i) * bget<T, T>(bPc, 0, 0, nb2.i[i]);
+    if constexpr (DEV) fst = fs - ph.template w<DIM>(i) * ps.template nbr<DIM>(nb2, i);
     dput<Q, T, S>(bFo, i, vx, fst);
   }
+}
+
+template <typename T, typename S, int DIM, bool LOCAL, bool FLUID = false, bool GMU = false>
+__device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
+                                             const Phys<T>& ph, uint32_t x) {
+  collide_node_src<T, S, DIM, LOCAL, FLUID, GMU>(a, ph, x, PhiGlobal<T>(a.phi_cur, a.g.N));
 }
 
 // Occupancy target of pass B (waves per SIMD); a build-time tuning knob.
@@ -1200,6 +1240,128 @@ __global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_collide_mixed(
   if (x >= a.g.N) return;
   if (c >> 31) collide_node<T, S, DIM, LOCAL, false>(a, ph, x);
   else collide_node<T, S, DIM, LOCAL, true>(a, ph, x);
+}
+
+// ---------------------------------------------------------------------------
+// Fused step (3D, distributions stored in the compute type): pass A and pass
+// B in one launch, without pass A's second read of the f set.  A block owns a
+// tile of ty x tx columns (y fastest) and a segment of zlen planes and marches
+// through them.  For every plane it first computes the start-of-step phi
+// (phi_node, as pass A) of the tile and of a one-node ring around it into a
+// four-plane LDS ring -- the ring nodes are the neighbouring tiles' own nodes,
+// recomputed here from the same pulls, so no block ever waits for another --
+// and then collides the tile's plane one behind, with the phi stencil read from
+// the ring.  Segments alternate their marching direction, so two neighbouring
+// segments reach the planes they share at the same time (both at their start
+// or both at their end) and the second reads of those planes hit the cache.
+// Each block adds up the phi of its own nodes in fp64 in a fixed order into
+// bpart[block]; k_fused_mass forms the step's mass factor from those sums.
+// ---------------------------------------------------------------------------
+struct FusedTile {
+  int ty, tx;        // tile extent along y (fastest) and x
+  int nty, ntx;      // tiles along y and x
+  int zlen;          // planes per z segment
+  int sy;            // ring row stride: ty + 2
+  int box;           // values per ring plane: (ty + 2) * (tx + 2) <= 2 * block size
+  double* bpart;     // one fp64 sum of phi per block
+};
+
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_fused(CollideArgs<T, T, 3> a,
+                                                               Phys<T> ph, FusedTile ft,
+                                                               T* __restrict__ phi_out) {
+  extern __shared__ __align__(16) unsigned char fused_lds[];
+  T* ring = reinterpret_cast<T*>(fused_lds);
+  const Grid& g = a.g;
+  const unsigned int L = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned int ntile = (unsigned int)(ft.nty * ft.ntx);
+  const int seg = (int)(L / ntile);
+  const int r = (int)(L - (unsigned int)seg * ntile);
+  const int txi = r / ft.nty, tyi = r - txi * ft.nty;
+  const int y0 = tyi * ft.ty, x0 = txi * ft.tx;
+  const int wy = min(ft.ty, g.ny - y0), wx = min(ft.tx, g.nx - x0);
+  const int z0 = seg * ft.zlen, z1 = min(g.nz, z0 + ft.zlen);
+  const int nown = z1 - z0;
+  const int dir = (seg & 1) ? -1 : 1;
+  const int zfirst = dir > 0 ? z0 : z1 - 1;
+  const int sy = ft.sy, box = ft.box;
+  const int tid = (int)threadIdx.x;
+
+  // ring positions of this thread (at most two per plane): in-plane base
+  // index, and flags 1 = inside the tile's box, 2 = a node of the tile itself
+  uint32_t qb[2];
+  int qf[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int q = tid + j * BS;
+    const int qx = q / sy, qy = q - qx * sy;
+    int yy = y0 - 1 + qy, xx = x0 - 1 + qx;
+    yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
+    xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
+    const bool in = q < box && qy <= wy + 1 && qx <= wx + 1;
+    const bool own = qy >= 1 && qy <= wy && qx >= 1 && qx <= wx;
+    qb[j] = (uint32_t)(xx * g.sx + yy * g.sy);
+    qf[j] = in ? (own ? 3 : 1) : 0;
+  }
+  // the node this thread collides: (cy, cx) of the tile
+  const int cx = tid / wy, cy = tid - cx * wy;
+  const bool cvalid = cx < wx;
+  const uint32_t cb = (uint32_t)((x0 + cx) * g.sx + (y0 + cy) * g.sy);
+  const int li = (cx + 1) * sy + (cy + 1);
+
+  const T cprev = (T)a.ctrl->c[a.slot_prev];
+  double acc = 0.0;
+  // march step s holds plane zfirst + (s - 1) dir (steps 0 and nown + 1 are
+  // the neighbouring segments' planes), in ring slot s & 3
+  auto plane = [&](int s) {
+    const int p = zfirst + (s - 1) * dir;
+    return p < 0 ? p + g.nz : (p >= g.nz ? p - g.nz : p);
+  };
+  auto phi_plane = [&](int s) {
+    const uint32_t zb = (uint32_t)plane(s) * (uint32_t)g.sz;
+    const bool own = s >= 1 && s <= nown;
+    T* slot = ring + (s & 3) * box;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (qf[j]) {
+        const uint32_t x = qb[j] + zb;
+        const T v = phi_node<T, T, 3, false>(g, ph, a.Fin, a.phi_prev, a.meta, a.eps, cprev, x);
+        slot[tid + j * BS] = v;
+        if (own && qf[j] == 3) {
+          at(phi_out, x) = v;
+          acc += (double)v;
+        }
+      }
+    }
+  };
+  phi_plane(0);
+  phi_plane(1);
+  for (int s = 1; s <= nown; ++s) {
+    phi_plane(s + 1);
+    // slot (s + 1) & 3 is written before this barrier and slots s - 1 .. s + 1
+    // are read after it; the next write, to slot (s + 2) & 3, is none of them
+    __syncthreads();
+    if (cvalid) {
+      const uint32_t x = cb + (uint32_t)plane(s) * (uint32_t)g.sz;
+      const PhiRing<T> pr{ring + ((s - dir) & 3) * box + li, ring + (s & 3) * box + li,
+                          ring + ((s + dir) & 3) * box + li, sy};
+      // one wave-uniform choice: the plain-fluid code when no node of the
+      // wave carries a geometry flag
+      const bool wall = __ballot((a.meta[x] & 0x0Fu) != 0) != 0ull;
+      if (wall) collide_node_src<T, T, 3, false, false, false>(a, ph, x, pr);
+      else collide_node_src<T, T, 3, false, true, false>(a, ph, x, pr);
+    }
+  }
+  __syncthreads();   // the ring is no longer read: reuse it for the block sum
+  const double tot = block_sum<BS>(acc, reinterpret_cast<double*>(fused_lds));
+  if (tid == 0) ft.bpart[L] = tot;
+}
+
+// Mass factor of a fused step from the block sums (fixed order), one block.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_fused_mass(const double* __restrict__ bpart,
+                                                   unsigned int n, Ctrl* ctrl, int slot) {
+  finalize_mass<BS>(bpart, n, ctrl, slot);
 }
 
 // Chunk classification: 1 when all 64 nodes of the chunk exist and are plain

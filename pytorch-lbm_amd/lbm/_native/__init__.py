@@ -103,6 +103,8 @@ _SIGS = {
     "lbm_set_occupancy": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "lbm_get_launch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "lbm_set_split": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbm_set_fused_tile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "lbm_get_fused_tile": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "lbm_state_bytes": (ctypes.c_size_t, [_vp]),
     "lbm_save_state": (ctypes.c_int, [_vp, _vp, _vp]),
     "lbm_load_state": (ctypes.c_int, [_vp, _vp, _vp]),

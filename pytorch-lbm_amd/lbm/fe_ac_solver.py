@@ -306,8 +306,22 @@ class AllenCahnSolver:
     def set_split(self, split):
         """Form of pass B: 0 (False) one launch over all nodes; 1 (True) two
         launches over the plain-fluid and the wall chunk lists; 2 one launch
-        over both lists, each XCD's range holding its wall chunks last."""
+        over both lists, each XCD's range holding its wall chunks last; 3 the
+        fused step (both passes in one launch; 3D, fp32/fp64 storage)."""
         nat.check(nat.lib.lbm_set_split(self._h, int(split)), "lbm_set_split")
+
+    def set_fused_tile(self, ty=0, tx=0, zlen=0):
+        """Tile of the fused step: ty x tx columns per workgroup, zlen planes
+        per segment (0: automatic)."""
+        nat.check(nat.lib.lbm_set_fused_tile(self._h, int(ty), int(tx), int(zlen)),
+                  "lbm_set_fused_tile")
+
+    def fused_tile(self):
+        """(ty, tx, tiles along y, tiles along x, zlen, segments) of the fused
+        step in use (zeros when another form is selected)."""
+        out = (ctypes.c_int * 6)()
+        nat.check(nat.lib.lbm_get_fused_tile(self._h, out), "lbm_get_fused_tile")
+        return tuple(out)
 
     def launch(self):
         """(block size A, block size B, cap A, cap B, form of pass B) of the step

@@ -227,12 +227,13 @@ def test_block_sizes_do_not_change_results():
     ref.step(12)
     for bsa, bsb, ca, cb, sp in ((128, 512, 0, 0, 0), (512, 128, 0, 0, 1), (256, 256, 6, 4, 0),
                                  (128, 256, 6, 5, 1), (256, 128, 0, 4, 2), (512, 256, 6, 0, 2),
-                                 (128, 512, 0, 5, 2)):
+                                 (128, 512, 0, 5, 2), (256, 512, 0, 0, 3), (128, 256, 6, 4, 3)):
         s, _, _ = gpu_solver("ridge162_readme_f32", torch.float32)
         s.set_block_sizes(bsa, bsb)
         s.set_occupancy(ca, cb)
         s.set_split(sp)
         assert s.launch() == (bsa, bsb, ca, cb, sp)
+        assert (s.fused_tile()[0] > 0) == (sp == 3)
         s.step(12)
         for key in ("phi", "u", "f", "g"):
             np.testing.assert_array_equal(np_(getattr(s, key)), np_(getattr(ref, key)))
@@ -248,7 +249,7 @@ def test_autotune_picks_valid_sizes():
     assert s.step_count == 27 * 4
 
 
-@pytest.mark.parametrize("split", [1, 2])
+@pytest.mark.parametrize("split", [1, 2, 3])
 @pytest.mark.parametrize("grid", [(19, 21, 17), (37, 29, 1)])
 def test_ragged_grid_matches_oracle(grid, split):
     """Node counts that are multiples of neither the 64-node wave nor the
@@ -256,7 +257,8 @@ def test_ragged_grid_matches_oracle(grid, split):
     the last block must stay out of every pull, sum and stencil.  Ridge with
     fraction, impact at theta = 162 with amplification 1.5, README
     construction; fp64 kernels against the fp64 oracle over 25 steps, with
-    pass B as two chunk-list launches and as the single mixed launch."""
+    pass B as two chunk-list launches, as the single mixed launch and (3D) as
+    the fused step."""
     need_gpu()
     from geometry.substrate import create_substrate_with_fraction
     from lbm.fe_ac_solver import AllenCahnSolver
@@ -291,3 +293,74 @@ def test_ragged_grid_matches_oracle(grid, split):
                 assert e < 1e-10, (step, key, e)
             assert rel(np_(s.f), o.f) < 1e-10
             assert rel(np_(s.g), o.g, floor=1e-5) < 1e-10
+
+
+def _ragged_fp32(grid=(19, 21, 17)):
+    from geometry.substrate import create_substrate_with_fraction
+    from lbm.fe_ac_solver import AllenCahnSolver
+    from lbm.fe_config import FEConfig
+    from lbm.fe_droplet import create_fe_droplet_with_impact
+    nx, ny, nz = grid
+    solid, frac = create_substrate_with_fraction(nx, ny, nz, substrate_type="ridge",
+                                                 R_star=1.0, R_d=5.0)
+    C0, _, u0 = create_fe_droplet_with_impact(nx, ny, nz, center=(nx / 2.0, ny / 2.0, 10.0),
+                                              radius=4.5, xi=4.0, rho_l=1.0, rho_g=1.0 / 828.0,
+                                              u_impact=(0.0, 0.0, -0.04))
+    m = {"grid": [nx, ny, nz], "theta": 162.0, "tau_h": None,
+         "kw": {"geo_amplification": 1.5, "geometric_wetting": True}}
+    cfg = FEConfig(**config_kwargs(m), device="cuda")
+    s = AllenCahnSolver(cfg, dtype=torch.float32, **solver_kwargs(m, None, "cuda"))
+    s.set_solid(solid, frac)
+    s.init_fields(C0, u0)
+    return s
+
+
+@pytest.mark.parametrize("bs,tile", [(256, (0, 0, 0)), (256, (1, 1, 1)), (256, (21, 1, 2)),
+                                     (128, (5, 3, 4)), (256, (7, 9, 17)), (256, (16, 16, 0)),
+                                     (512, (21, 19, 3))])
+def test_fused_tiles_bit_identical(bs, tile):
+    """The fused step (pass A and pass B in one launch, phi stencil from an
+    LDS ring around each tile) gives the same fp32 states as the two-pass step
+    for every tile shape: single columns, whole rows, tiles larger than the
+    grid's remainder (ragged edge tiles), one plane per segment and one
+    segment for the whole height, on a 19 x 21 x 17 grid whose periodic wraps
+    cross every tile edge.  Also replayed from a captured graph."""
+    need_gpu()
+    print(f"[gpu] fused tile bs {bs} {tile}", flush=True)
+    ref = _ragged_fp32()
+    ref.set_split(2)
+    s = _ragged_fp32()
+    s.set_split(3)
+    s.set_block_sizes(256, bs)
+    s.set_fused_tile(*tile)
+    plan = s.fused_tile()
+    assert plan[0] > 0 and plan[0] * plan[1] <= bs
+    if tile[0]:
+        assert plan[:2] == (min(tile[0], 21), min(tile[1], 19))
+    ref.step(1)
+    s.step(1)            # first step after init_fields: the chunk-list launches
+    ref.step(6)
+    s.step(6)
+    for key in ("phi", "u", "f", "g", "P"):
+        np.testing.assert_array_equal(np_(getattr(s, key)), np_(getattr(ref, key)))
+    s.capture_graph(2)
+    for _ in range(2):
+        ref.step(2)
+        s.run_graph(1)
+        torch.cuda.synchronize()
+        for key in ("phi", "u", "f", "g"):
+            np.testing.assert_array_equal(np_(getattr(s, key)), np_(getattr(ref, key)))
+    m_ref, m_s = ref.mass()[1], s.mass()[1]
+    assert abs(m_s - m_ref) <= 1e-12 * abs(m_ref)
+
+
+def test_fused_tile_rejects_oversized_tiles():
+    need_gpu()
+    from lbm._native import LbmError
+    s = _ragged_fp32()
+    s.set_split(3)
+    s.set_block_sizes(256, 256)
+    with pytest.raises(LbmError):
+        s.set_fused_tile(21, 19, 0)      # 399 columns > 256 threads
+    assert s.fused_tile()[0] > 0         # the previous (automatic) plan stays
+    s.step(3)
