@@ -109,6 +109,18 @@ def cpu_baseline(inputs, seconds=15.0, threads=None):
                       f"{threads} threads, {dt:.1f} s on {cpu_model()}"}
 
 
+def hbm_stream(bytes_=1 << 30, reps=5):
+    """Streaming bandwidth of the device in TB/s (lbm_hbm_stream: read, write
+    and copy kernels over 1 GiB buffers, best of three grid sizes): the
+    practical ceiling the step's measured traffic is compared with."""
+    import ctypes
+    from lbm import _native as nat
+    out = (ctypes.c_double * 6)()
+    nat.check(nat.lib.lbm_hbm_stream(bytes_, reps, nat.stream_ptr(), out), "lbm_hbm_stream")
+    return dict(zip(("read", "read_nt", "write", "write_nt", "copy", "copy_nt"),
+                    (round(v, 3) for v in out)))
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -462,6 +474,7 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    stream_tbs = hbm_stream()
     s, inputs = cfg2(args.storage)
     n = 150 * 150 * 107
     sixteen = args.storage in ("bf16", "fp16")
@@ -537,6 +550,7 @@ def main():
                      "bytes_per_lu": b_coll, "avg_ms": round(tb, 5),
                      "launch": dict(zip(("block_phi", "block_collide", "cap_phi",
                                          "cap_collide", "split"), s.launch()))},
+        "hbm_stream_TBps": stream_tbs,
         "timing": (f"replays of a captured {g}-step graph" if g else "eager launches"),
         "kernels_ms": {"k_phi": round(ta, 5), "k_collide": round(tb, 5),
                        "step_events": round(tstep, 5)},

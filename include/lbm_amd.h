@@ -296,6 +296,14 @@ int lbm_base_radius(lbm_solver* s, double cx, double cy, double* out, void* stre
  * tensor divisions are IEEE). */
 int lbm_check_division(float d, unsigned long long* mismatches, void* stream);
 
+/* Streaming bandwidth of the device (the practical ceiling the step's traffic
+ * is compared with): best over three grid sizes of `reps` passes over
+ * `bytes` (>= 1 MiB; two buffers of that size are allocated and freed), in
+ * TB/s: out6 = {read, read non-temporal, write, write non-temporal, copy,
+ * copy non-temporal} (a copy counts its reads and its writes).  Synchronises
+ * `stream`.  No counterpart in the reference. */
+int lbm_hbm_stream(size_t bytes, int reps, void* stream, double* out6);
+
 /* ------------------------------------------------------------------------
  * Batch engine: many cases of uneven grid size advanced together, one launch
  * per pass per step for all of them (no reference counterpart: the reference

@@ -461,6 +461,92 @@ int lbm_check_division(float d, unsigned long long* mismatches, void* stream) {
   return LBM_OK;
 }
 
+// Streaming kernels for lbm_hbm_stream: 16 bytes per lane, grid-stride,
+// default or non-temporal accesses.
+typedef float f4v __attribute__((ext_vector_type(4)));
+template <int NT>
+__global__ __launch_bounds__(256) void k_stream_read(const f4v* __restrict__ a, size_t n,
+                                                     float* __restrict__ sink) {
+  float acc = 0.f;
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256ull) {
+    const f4v v = NT ? __builtin_nontemporal_load(a + i) : a[i];
+    acc += (v.x + v.y) + (v.z + v.w);
+  }
+  if (acc == -1.0f) sink[0] = acc;   // never true for the zero-filled source; keeps the loads
+}
+template <int NT>
+__global__ __launch_bounds__(256) void k_stream_write(f4v* __restrict__ a, size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256ull) {
+    const f4v v = {(float)i, 1.f, 2.f, 3.f};
+    if (NT) __builtin_nontemporal_store(v, a + i);
+    else a[i] = v;
+  }
+}
+template <int NT>
+__global__ __launch_bounds__(256) void k_stream_copy(const f4v* __restrict__ a,
+                                                     f4v* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256ull) {
+    const f4v v = NT ? __builtin_nontemporal_load(a + i) : a[i];
+    if (NT) __builtin_nontemporal_store(v, b + i);
+    else b[i] = v;
+  }
+}
+
+int lbm_hbm_stream(size_t bytes, int reps, void* stream, double* out6) {
+  if (!out6) return fail(LBM_EINVAL, "null argument");
+  if (bytes < (1u << 20) || reps < 1) return fail(LBM_EINVAL, "bytes >= 1 MiB and reps >= 1");
+  bytes &= ~(size_t)15;
+  const size_t n = bytes / 16;
+  hipStream_t st = (hipStream_t)stream;
+  void *a = nullptr, *b = nullptr, *sink = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = LBM_OK;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (rc == LBM_OK && e != hipSuccess) rc = fail(LBM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  chk(hipMalloc(&a, bytes), "hipMalloc");
+  chk(hipMalloc(&b, bytes), "hipMalloc");
+  chk(hipMalloc(&sink, 64), "hipMalloc");
+  chk(hipEventCreate(&e0), "hipEventCreate");
+  chk(hipEventCreate(&e1), "hipEventCreate");
+  if (rc == LBM_OK) chk(hipMemsetAsync(a, 0, bytes, st), "hipMemsetAsync");
+  if (rc == LBM_OK) chk(hipMemsetAsync(b, 0, bytes, st), "hipMemsetAsync");
+  const unsigned int grids[3] = {1024u, 4096u, 16384u};
+  for (int kind = 0; kind < 6 && rc == LBM_OK; ++kind) {
+    double best = 0.0;
+    for (unsigned int G : grids) {
+      auto launch = [&]() {
+        switch (kind) {
+          case 0: k_stream_read<0><<<G, 256, 0, st>>>((const f4v*)a, n, (float*)sink); break;
+          case 1: k_stream_read<1><<<G, 256, 0, st>>>((const f4v*)a, n, (float*)sink); break;
+          case 2: k_stream_write<0><<<G, 256, 0, st>>>((f4v*)b, n); break;
+          case 3: k_stream_write<1><<<G, 256, 0, st>>>((f4v*)b, n); break;
+          case 4: k_stream_copy<0><<<G, 256, 0, st>>>((const f4v*)a, (f4v*)b, n); break;
+          default: k_stream_copy<1><<<G, 256, 0, st>>>((const f4v*)a, (f4v*)b, n); break;
+        }
+      };
+      launch();
+      chk(hipEventRecord(e0, st), "hipEventRecord");
+      for (int r = 0; r < reps; ++r) launch();
+      chk(hipEventRecord(e1, st), "hipEventRecord");
+      chk(hipEventSynchronize(e1), "hipEventSynchronize");
+      float ms = 0.f;
+      chk(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      if (rc != LBM_OK) break;
+      const double moved = (kind >= 4 ? 2.0 : 1.0) * (double)(n * 16) * reps;
+      const double tbs = moved / (ms * 1e-3) / 1e12;
+      if (tbs > best) best = tbs;
+    }
+    out6[kind] = best;
+  }
+  if (rc == LBM_OK) chk(hipGetLastError(), "stream kernel launch");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (void* q : {a, b, sink})
+    if (q) (void)hipFree(q);
+  return rc;
+}
+
 int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide) {
   if (!s) return fail(LBM_EINVAL, "null solver");
   REFUSE_BATCHED(s);

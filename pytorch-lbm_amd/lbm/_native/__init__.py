@@ -99,6 +99,8 @@ _SIGS = {
     "lbm_check_division": (ctypes.c_int, [ctypes.c_float, ctypes.POINTER(ctypes.c_ulonglong),
                                           _vp]),
     "lbm_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "lbm_hbm_stream": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, _vp,
+                                      ctypes.POINTER(ctypes.c_double)]),
     "lbm_set_block_sizes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "lbm_set_occupancy": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "lbm_get_launch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),

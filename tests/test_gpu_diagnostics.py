@@ -76,3 +76,19 @@ def test_reference_measure_helpers_on_the_solver():
         assert 0.0 <= theta <= 180.0
     assert mon.history["step"] == [5, 10, 15]
     assert mon.history["mass"][-1] == float(s.rho.cpu().numpy().sum())
+
+
+def test_hbm_stream_probe():
+    """The streaming-bandwidth probe returns plausible MI355X figures: every
+    kind between 1 and 8 TB/s (the HBM3E peak), reads faster than writes."""
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import ctypes
+    from lbm import _native as nat
+    out = (ctypes.c_double * 6)()
+    nat.check(nat.lib.lbm_hbm_stream(1 << 30, 3, nat.stream_ptr(), out), "lbm_hbm_stream")
+    v = list(out)
+    print(f"[gpu] stream TB/s read {v[0]:.2f}/{v[1]:.2f} write {v[2]:.2f}/{v[3]:.2f} "
+          f"copy {v[4]:.2f}/{v[5]:.2f}", flush=True)
+    assert all(1.0 < x < 8.0 for x in v), v
+    assert max(v[0], v[1]) > max(v[2], v[3])
