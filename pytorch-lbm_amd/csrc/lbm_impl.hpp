@@ -498,7 +498,7 @@ int step_impl(lbm_solver* s, int nsteps, hipStream_t st, hipEvent_t* ev = nullpt
     if (s->local) {                                                                         \
       if (np) k_collide_chunks<T, S, DIM, B, true, true><<<gp, B, s->ldsB, st>>>(a, ph, cp, np, 1); \
       if (nw) k_collide_chunks<T, S, DIM, B, true, false><<<gw, B, s->ldsB, st>>>(a, ph, cw, nw, np == 0); \
-    } else if (s->split == 2) {                                                             \
+    } else if (s->split >= 2) {                                                             \
       const int k = B == 128 ? 0 : (B == 256 ? 1 : 2);                                      \
       k_collide_mixed<T, S, DIM, B, false><<<s->n_mixed[k] / (B / 64), B, s->ldsB, st>>>(   \
           a, ph, s->mixed[k], s->n_mixed[k]);                                                \

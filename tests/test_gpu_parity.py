@@ -295,7 +295,7 @@ def test_ragged_grid_matches_oracle(grid, split):
             assert rel(np_(s.g), o.g, floor=1e-5) < 1e-10
 
 
-def _ragged_fp32(grid=(19, 21, 17)):
+def _ragged_fp32(grid=(19, 21, 17), storage=None):
     from geometry.substrate import create_substrate_with_fraction
     from lbm.fe_ac_solver import AllenCahnSolver
     from lbm.fe_config import FEConfig
@@ -309,10 +309,26 @@ def _ragged_fp32(grid=(19, 21, 17)):
     m = {"grid": [nx, ny, nz], "theta": 162.0, "tau_h": None,
          "kw": {"geo_amplification": 1.5, "geometric_wetting": True}}
     cfg = FEConfig(**config_kwargs(m), device="cuda")
-    s = AllenCahnSolver(cfg, dtype=torch.float32, **solver_kwargs(m, None, "cuda"))
+    s = AllenCahnSolver(cfg, dtype=torch.float32, storage=storage, **solver_kwargs(m, None, "cuda"))
     s.set_solid(solid, frac)
     s.init_fields(C0, u0)
     return s
+
+
+@pytest.mark.parametrize("storage", ["bf16", "fp16"])
+def test_fused_form_elsewhere_runs_form_2(storage):
+    """Form 3 on a solver the fused step does not cover (16-bit storage) runs
+    the single mixed pass-B launch: no fused plan, identical states."""
+    need_gpu()
+    a = _ragged_fp32(storage=storage)
+    a.set_split(2)
+    b = _ragged_fp32(storage=storage)
+    b.set_split(3)
+    assert b.fused_tile() == (0, 0, 0, 0, 0, 0)
+    a.step(6)
+    b.step(6)
+    for key in ("phi", "u", "f", "g"):
+        np.testing.assert_array_equal(np_(getattr(b, key)), np_(getattr(a, key)))
 
 
 @pytest.mark.parametrize("bs,tile", [(256, (0, 0, 0)), (256, (1, 1, 1)), (256, (21, 1, 2)),
