@@ -560,6 +560,7 @@ int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide) {
   s->ldsB = lds_for_cap(s->bsB, s->capB, 1024);   // finalize_mass's 128 doubles
   drop_graph(s);
   s->f_blocks = 0;
+  s->f_ty = s->f_tx = s->f_nty = s->f_ntx = s->f_zlen = s->f_nseg = 0;
   if (s->split == 3 && fused_ok(s)) {
     int pl[6];
     if (!fused_plan(s->g, s->bsB, s->freq_ty, s->freq_tx, s->freq_zlen, pl))

@@ -144,9 +144,22 @@ int main() {
     int launch[5] = {0, 0, 0, 0, 0};
     OK(lbm_get_launch(s, launch));
     CHECK(launch[4] == 2, "fp32 default pass-B form %d", launch[4]);
-    CHECK(lbm_set_split(s, 3) == LBM_EINVAL && lbm_set_split(s, -1) == LBM_EINVAL,
+    CHECK(lbm_set_split(s, 4) == LBM_EINVAL && lbm_set_split(s, -1) == LBM_EINVAL,
           "bad pass-B form accepted");
+    // the fused form plans a tile; an oversized tile is refused and the plan stays
+    int plan[6] = {0, 0, 0, 0, 0, 0};
+    OK(lbm_set_split(s, 3));
+    OK(lbm_get_fused_tile(s, plan));
+    CHECK(plan[0] > 0 && plan[1] > 0 && plan[0] * plan[1] <= 256 && plan[5] >= 1,
+          "fused plan %d %d %d %d %d %d", plan[0], plan[1], plan[2], plan[3], plan[4], plan[5]);
+    CHECK(lbm_set_fused_tile(s, 64, 64, 0) == LBM_EINVAL, "oversized fused tile accepted");
+    CHECK(lbm_set_fused_tile(s, -1, 0, 0) == LBM_EINVAL, "negative fused tile accepted");
+    int plan2[6] = {0, 0, 0, 0, 0, 0};
+    OK(lbm_get_fused_tile(s, plan2));
+    CHECK(plan2[0] == plan[0] && plan2[1] == plan[1], "fused plan changed by a refused tile");
     OK(lbm_set_split(s, 1));
+    OK(lbm_get_fused_tile(s, plan2));
+    CHECK(plan2[0] == 0, "fused plan kept after leaving form 3");
     OK(lbm_get_launch(s, launch));
     CHECK(launch[4] == 1, "pass-B form not set: %d", launch[4]);
     OK(lbm_destroy(s));
