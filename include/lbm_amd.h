@@ -180,9 +180,12 @@ int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide);
 int lbm_set_split(lbm_solver* s, int split);
 
 /* Tile of the fused step: ty x tx columns (y, x) per workgroup, zlen planes
- * per segment; 0 chooses automatically (the default).  LBM_EINVAL when
- * ty * tx exceeds the pass-B block size or (ty + 2) * (tx + 2) twice it.
- * Invalidates a captured graph.  No counterpart in the reference. */
+ * per segment; 0 chooses automatically (the default; sizes above the grid's
+ * are clipped to it).  Checked whenever the fused form is (re)planned -- here
+ * when form 3 is selected, and by lbm_set_split / lbm_set_block_sizes /
+ * lbm_set_occupancy: LBM_EINVAL, with the previous setting kept, when ty * tx
+ * exceeds the pass-B block size or (ty + 2) * (tx + 2) twice it.  Invalidates
+ * a captured graph.  No counterpart in the reference. */
 int lbm_set_fused_tile(lbm_solver* s, int ty, int tx, int zlen);
 
 /* Fused-step plan in use: out6 = {ty, tx, tiles along y, tiles along x,

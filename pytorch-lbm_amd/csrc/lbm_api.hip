@@ -552,31 +552,36 @@ int lbm_set_block_sizes(lbm_solver* s, int bs_phi, int bs_collide) {
   REFUSE_BATCHED(s);
   auto ok = [](int b) { return b == 128 || b == 256 || b == 512; };
   if (!ok(bs_phi) || !ok(bs_collide)) return fail(LBM_EINVAL, "block size must be 128, 256 or 512");
+  // the fused form's plan (and its partials buffer) first: a tile that does
+  // not fit leaves every setting as it was
+  const bool fused = s->split == 3 && fused_ok(s);
+  int pl[6] = {0, 0, 0, 0, 0, 0};
+  unsigned int nbF = 0;
+  if (fused) {
+    if (!fused_plan(s->g, bs_collide, s->freq_ty, s->freq_tx, s->freq_zlen, pl))
+      return fail(LBM_EINVAL, "fused tile does not fit the pass-B block size "
+                              "(ty * tx <= block and (ty + 2) * (tx + 2) <= 2 * block)");
+    nbF = (unsigned int)(pl[2] * pl[3] * pl[5]);
+    if (nbF > s->fpart_cap) {
+      double* np = nullptr;
+      HIPCHK(hipMalloc(&np, (size_t)nbF * sizeof(double)));
+      if (s->fpart) (void)hipFree(s->fpart);
+      s->fpart = np;
+      s->fpart_cap = nbF;
+    }
+  }
+  drop_graph(s);
   s->bsA = bs_phi;
   s->bsB = bs_collide;
   s->nbA = (unsigned int)((s->N + bs_phi - 1) / bs_phi);
   s->nbB = (unsigned int)((s->N + bs_collide - 1) / bs_collide);
   s->ldsA = lds_for_cap(s->bsA, s->capA, 0);
   s->ldsB = lds_for_cap(s->bsB, s->capB, 1024);   // finalize_mass's 128 doubles
-  drop_graph(s);
-  s->f_blocks = 0;
-  s->f_ty = s->f_tx = s->f_nty = s->f_ntx = s->f_zlen = s->f_nseg = 0;
-  if (s->split == 3 && fused_ok(s)) {
-    int pl[6];
-    if (!fused_plan(s->g, s->bsB, s->freq_ty, s->freq_tx, s->freq_zlen, pl))
-      return fail(LBM_EINVAL, "fused tile does not fit the pass-B block size "
-                              "(ty * tx <= block and (ty + 2) * (tx + 2) <= 2 * block)");
-    const unsigned int nb = (unsigned int)(pl[2] * pl[3] * pl[5]);
-    if (nb > s->fpart_cap) {
-      if (s->fpart) (void)hipFree(s->fpart);
-      s->fpart = nullptr;
-      s->fpart_cap = 0;
-      HIPCHK(hipMalloc(&s->fpart, (size_t)nb * sizeof(double)));
-      s->fpart_cap = nb;
-    }
-    s->f_ty = pl[0]; s->f_tx = pl[1]; s->f_nty = pl[2]; s->f_ntx = pl[3];
-    s->f_zlen = pl[4]; s->f_nseg = pl[5];
-    s->f_blocks = nb;
+  s->f_ty = pl[0]; s->f_tx = pl[1]; s->f_nty = pl[2]; s->f_ntx = pl[3];
+  s->f_zlen = pl[4]; s->f_nseg = pl[5];
+  s->f_blocks = nbF;
+  s->ldsF = 0;
+  if (fused) {
     // four ring planes; the occupancy cap pads the same dynamic allocation
     const unsigned int ring = 4u * (unsigned int)((pl[0] + 2) * (pl[1] + 2)) * (unsigned int)s->esz;
     const unsigned int tot = lds_for_cap(s->bsB, s->capB, 0);
@@ -593,10 +598,9 @@ int lbm_set_fused_tile(lbm_solver* s, int ty, int tx, int zlen) {
   s->freq_ty = ty;
   s->freq_tx = tx;
   s->freq_zlen = zlen;
-  const int rc = lbm_set_block_sizes(s, s->bsA, s->bsB);
+  const int rc = lbm_set_block_sizes(s, s->bsA, s->bsB);   // changes nothing on failure
   if (rc) {
     s->freq_ty = o[0]; s->freq_tx = o[1]; s->freq_zlen = o[2];
-    (void)lbm_set_block_sizes(s, s->bsA, s->bsB);
   }
   return rc;
 }
@@ -613,9 +617,15 @@ int lbm_set_occupancy(lbm_solver* s, int cap_phi, int cap_collide) {
   REFUSE_BATCHED(s);
   if (cap_phi < 0 || cap_phi > 8 || cap_collide < 0 || cap_collide > 8)
     return fail(LBM_EINVAL, "occupancy cap must be 0 (none) or 1..8 waves per SIMD");
+  const int oa = s->capA, ob = s->capB;
   s->capA = cap_phi;
   s->capB = cap_collide;
-  return lbm_set_block_sizes(s, s->bsA, s->bsB);
+  const int rc = lbm_set_block_sizes(s, s->bsA, s->bsB);
+  if (rc) {
+    s->capA = oa;
+    s->capB = ob;
+  }
+  return rc;
 }
 
 int lbm_set_split(lbm_solver* s, int split) {
@@ -624,11 +634,8 @@ int lbm_set_split(lbm_solver* s, int split) {
   if (split < 0 || split > 3) return fail(LBM_EINVAL, "split must be 0, 1, 2 or 3");
   const int old = s->split;
   s->split = split;
-  const int rc = lbm_set_block_sizes(s, s->bsA, s->bsB);
-  if (rc) {
-    s->split = old;
-    (void)lbm_set_block_sizes(s, s->bsA, s->bsB);
-  }
+  const int rc = lbm_set_block_sizes(s, s->bsA, s->bsB);   // changes nothing on failure
+  if (rc) s->split = old;
   return rc;
 }
 

@@ -378,5 +378,13 @@ def test_fused_tile_rejects_oversized_tiles():
     s.set_block_sizes(256, 256)
     with pytest.raises(LbmError):
         s.set_fused_tile(21, 19, 0)      # 399 columns > 256 threads
-    assert s.fused_tile()[0] > 0         # the previous (automatic) plan stays
+    plan = s.fused_tile()
+    assert plan[0] > 0                   # the previous (automatic) plan stays
+    s.set_fused_tile(16, 16, 0)          # fits 256 threads ...
+    with pytest.raises(LbmError):
+        s.set_block_sizes(256, 128)      # ... but not 128: refused, nothing changes
+    assert s.launch()[1] == 256 and s.fused_tile()[:2] == (16, 16)
+    with pytest.raises(LbmError):
+        s.set_occupancy(6, 9)
+    assert s.launch()[2:4] == (0, 0)
     s.step(3)
