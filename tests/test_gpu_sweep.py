@@ -133,7 +133,7 @@ def test_sweep_fp32_batch_equals_solver():
 
 def test_sweep_spread_within_1pct_of_fp64():
     """Max spread factor of the batched bf16 cases against fp64 runs of the
-    same cases (north_star's 1% gate)."""
+    same cases (the 1% precision gate)."""
     need_gpu()
     print("[gpu] sweep: bf16 spread vs fp64", flush=True)
     by_tag = {c.tag: c for c in cases()}
