@@ -4,9 +4,9 @@
 value: MLUPS of the plain fp32 D3Q19 step on the 150x150x107 ridge impact,
 with `roofline` (the collide kernel against the 8 TB/s HBM peak), the step's
 % of the HBM roofline at 356 algorithmic B/LU, and beside it
-  sweep    the 48-case uneven impact sweep (1500 steps each) batched in bf16
-           storage against the same cases run one after another in the plain
-           fp32 form: solves/hour and the speed-up;
+  sweep    the 48-case uneven impact sweep (1500 steps each) batched in fp16
+           storage (--sweep-storage) against the same cases run one after
+           another in the plain fp32 form: solves/hour and the speed-up;
   serve    a seeded Poisson stream (4 requests/s for 20 s) of impact solves
            served with continuous batching: p50/p99 time to solution and the
            aggregate MLUPS;
@@ -24,6 +24,14 @@ tau_h=1.0; ridge R*=1, R_d=22.5 with solid fraction; droplet R=22.5 at
 (75,75,80), u=(0,0,-0.05).  Inputs are resident in HBM before the timed
 region; a "step" is one solver.step() over all 2,407,500 nodes (solid
 included).  Only one GPU is supported.
+
+Timing: after the autotuner and the W warm-up steps, the timed steps are
+captured into a graph, which is first replayed untimed for at least
+SUSTAIN_STEPS steps (`sustain_steps` in the line): the device reaches its
+sustained-load speed only after about 0.1 s of stepping (measured, 20 timed
+steps right after capture against after 500 untimed steps: 175.1-176.9 vs
+173.5 us/step on one box, 170.7-174.1 vs 166.3-167.3 on another).  The timed
+region is then exactly K steps between two synchronisations.
 """
 import argparse
 import json
@@ -418,6 +426,9 @@ def serve_bench(duration=20.0, rate=4.0, seed=5, storage="bf16", slots=32, chunk
     }
 
 
+SUSTAIN_STEPS = 500
+
+
 def graph_chunk(steps, most=50):
     """Largest even g <= most dividing steps (steps - 1 when steps is odd);
     0 when fewer than two steps are timed."""
@@ -507,11 +518,14 @@ def main():
     s.step(max(args.warmup, 1))
     # the timed steps run as replays of a captured graph of g steps (even,
     # dividing the step count; an odd count runs its last step eagerly),
-    # replayed once untimed first
+    # replayed untimed for at least SUSTAIN_STEPS steps first
     g = 0 if args.eager else graph_chunk(args.steps)
+    sustain = 0
     if g:
         s.capture_graph(g)
-        s.run_graph(1)
+        reps = max(1, -(-SUSTAIN_STEPS // g))
+        s.run_graph(reps)
+        sustain = reps * g
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -565,6 +579,7 @@ def main():
                                          "cap_collide", "split"), s.launch()))},
         "hbm_stream_TBps": stream_tbs,
         "timing": (f"replays of a captured {g}-step graph" if g else "eager launches"),
+        "sustain_steps": sustain,
         "kernels_ms": {"k_phi": round(ta, 5), "k_collide": round(tb, 5),
                        "step_events": round(tstep, 5)},
         "mass_drift": abs(mass - mass0) / mass0,
