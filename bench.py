@@ -606,7 +606,7 @@ def main():
         rec["serve"] = {k: sv[k] for k in ("p50_time_to_solution_s", "p99_time_to_solution_s",
                                            "aggregate_MLUPS", "offered_MLUPS", "requests",
                                            "rate_per_s", "p99_queue_wait_s", "peak_active",
-                                           "storage", "any_nan")}
+                                           "storage", "any_nan", "cpu_baseline")}
         rec["serve"]["workload"] = ("seeded Poisson stream, 4 requests/s for 20 s, grids "
                                     "64^3 .. 256x256x160, 500-3000 steps, continuous batching")
     if not args.no_cpu:
