@@ -293,10 +293,12 @@ int lbm_contact_radius(lbm_solver* s, double cx, double cy, double* out, void* s
  * this radius follows the drop's own base instead.  Synchronises `stream`. */
 int lbm_base_radius(lbm_solver* s, double cx, double cy, double* out, void* stream);
 
-/* Self-check of the kernels' division by a constant: counts the fp32 inputs
- * (all 2^32 bit patterns, NaNs excepted) whose quotient by d differs from IEEE
- * division.  Synchronises `stream`.  No counterpart in the reference (whose
- * tensor divisions are IEEE). */
+/* Self-check of the kernels' division by a divisor d > 0: counts the fp32
+ * inputs (all 2^32 bit patterns, NaNs excepted) whose quotient by d differs
+ * from IEEE division, over both division forms the kernels use (the general
+ * one always, the one-product form unless d is an even integer).
+ * Synchronises `stream`.  No counterpart in the reference (whose tensor
+ * divisions are IEEE). */
 int lbm_check_division(float d, unsigned long long* mismatches, void* stream);
 
 /* Streaming bandwidth of the device (the practical ceiling the step's traffic

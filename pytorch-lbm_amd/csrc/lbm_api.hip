@@ -425,33 +425,37 @@ int lbm_profile_steps(lbm_solver* s, int nsteps, void* stream, double* out) {
   return rc;
 }
 
-// Exhaustive check of divk against IEEE division for one fp32 divisor: every
-// one of the 2^32 fp32 bit patterns x (NaN inputs excepted) must give the
+// Exhaustive check of the kernels' division forms against IEEE division for
+// one fp32 divisor: for every one of the 2^32 fp32 bit patterns x (NaN inputs
+// excepted) divk_any -- and divk, unless d is an even integer -- must give the
 // bit-identical quotient.
-__global__ void k_check_div(float d, float r, unsigned long long* bad) {
+__global__ void k_check_div(float d, double y, int plain, unsigned long long* bad) {
   const unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned int cnt = 0;
   for (unsigned int j = 0; j < 64; ++j) {
     const unsigned int bits = (unsigned int)(t * 64ull + j);
     const float x = __builtin_bit_cast(float, bits);
     if (x != x) continue;
-    const float a = divk(x, d, r);
-    const float b = x / d;
-    if (__builtin_bit_cast(unsigned int, a) != __builtin_bit_cast(unsigned int, b)) ++cnt;
+    const unsigned int b = __builtin_bit_cast(unsigned int, x / d);
+    if (__builtin_bit_cast(unsigned int, divk_any(x, d, y)) != b) ++cnt;
+    if (plain && __builtin_bit_cast(unsigned int, divk(x, d, y)) != b) ++cnt;
   }
   if (cnt) atomicAdd(bad, (unsigned long long)cnt);
 }
 
 int lbm_check_division(float d, unsigned long long* mismatches, void* stream) {
   if (!mismatches) return fail(LBM_EINVAL, "null argument");
+  if (!(d > 0.0f) || !std::isfinite(d)) return fail(LBM_EINVAL, "divisor must be positive and finite");
   hipStream_t st = (hipStream_t)stream;
   unsigned long long* dev = nullptr;
   HIPCHK(hipMalloc(&dev, sizeof(unsigned long long)));
   hipError_t e = hipMemsetAsync(dev, 0, sizeof(unsigned long long), st);
-  const float r = 1.0f / d;
+  const double y = 1.0 / (double)d;
+  // divk is claimed exact for every divisor that is not an even integer
+  const int plain = !(d >= 2.0f && std::floor(d * 0.5f) == d * 0.5f);
   if (e == hipSuccess) {
     // 2^32 values, 64 per thread, 256 threads per block -> 2^18 blocks
-    k_check_div<<<1u << 18, 256, 0, st>>>(d, r, dev);
+    k_check_div<<<1u << 18, 256, 0, st>>>(d, y, plain, dev);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(mismatches, dev, sizeof(*mismatches), hipMemcpyDeviceToHost, st);

@@ -161,8 +161,9 @@ template <typename T, int DIM> inline Phys<T> make_phys(const lbm_solver* s) {
   const double cs2 = 1.0 / 3.0;
   ph.cs2 = TT<T>(cs2);
   ph.two_cs2 = TT<T>(2.0 * cs2);
-  ph.rcp_cs2 = T(1) / ph.cs2;
-  ph.rcp_two_cs2 = T(1) / ph.two_cs2;
+  // fp64 reciprocals of the divisors as the run holds them (the divk forms)
+  ph.rcp_cs2 = 1.0 / (double)ph.cs2;
+  ph.rcp_two_cs2 = 1.0 / (double)ph.two_cs2;
   for (int i = 0; i < Q; ++i) {
     const int c = wclass<DIM>(i);
     ph.wc[c] = (T)w32<DIM>(i);
@@ -179,7 +180,9 @@ template <typename T, int DIM> inline Phys<T> make_phys(const lbm_solver* s) {
   ph.drho = TT<T>(p.rho_l - p.rho_g);
   ph.visc = TT<T>((0.5 - p.tau_g) / p.tau_g);
   ph.xi = TT<T>(p.xi);
-  ph.rcp_xi = T(1) / ph.xi;
+  ph.rcp_xi = 1.0 / (double)ph.xi;
+  // rho_eff cs2 stays below max(rho_l, rho_g) cs2 (phi is clamped to [0, 1])
+  ph.rc_small = std::max(p.rho_l, p.rho_g) * cs2 < 1.9;
   ph.ac_scale = TT<T>(p.ac_scale);
   ph.wet = s->wet;
   ph.mu_wall_add = TT<T>(s->mu_wall * p.mu_wall_scale);

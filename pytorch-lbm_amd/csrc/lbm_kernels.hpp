@@ -180,23 +180,44 @@ __device__ __forceinline__ void dput(const Buf& b, int i, uint32_t vo, T v) {
                             Stor<S>::template from<T>(v));
 }
 
-// Correctly rounded division by a divisor d whose reciprocal r = RN(1/d) is
-// known: q = RN(x r), residual e = x - q d exactly (one FMA), q' = RN(q + e r)
-// (Markstein's theorem: a faithful q plus one FMA correction with the
-// correctly rounded reciprocal gives RN(x/d), barring over/underflow).  The
-// fp32 case is checked exhaustively against IEEE division for the constant
-// divisors by lbm_check_division; fp64 keeps the IEEE division.
-__device__ __forceinline__ float divk(float x, float d, float r) {
-  const float ax = __builtin_fabsf(x);
-  // outside [2^-100, 2^100] (and for 0, inf, NaN) the residual or the product
-  // can under/overflow: use the IEEE division there (rare; exhaustively
-  // checked: the two paths together equal IEEE division for every input)
-  if (__builtin_expect(!(ax >= 0x1p-100f && ax <= 0x1p100f), 0)) return x / d;
-  const float q = x * r;
-  const float e = __builtin_fmaf(-q, d, x);
-  return __builtin_fmaf(e, r, q);
+// Correctly rounded fp32 division x / d by a divisor whose fp64 reciprocal
+// y = RN64(1/d) is known, without a branch or a range test (fp32 and fp64
+// FMAs issue at the same rate on gfx950 when packed fp32 math is off).
+//
+// divk: RN32(RN64(x y)).  x y is within 2^-52 (relative) of x / d.  Write
+// x = a 2^ex, d = b 2^ed (a, b integers below 2^24).  Unless x / d lies
+// exactly on a rounding boundary m of fp32, x - m d is a nonzero multiple of
+// 2^min(ex, ed - 150), so x / d is at least 2^-49 (relative) from every
+// boundary of a normal result and at least min(2^(ex-ed-24), 2^-174) from
+// every boundary of a subnormal one -- both beyond the 2^-52 -- and rounding
+// x y to fp32 gives RN32(x / d).  x / d = m exactly needs m d to be an fp32
+// value with m a midpoint (odd multiple of 2^-150 or 25 significant bits),
+// which happens only for d an even integer.  So divk is exact for every fp32
+// x (zeros, subnormals and infinities included) whenever d is not an even
+// integer: cs2, 2 cs2 and rho_eff cs2 below 2.
+// divk_any: any d > 0.  Markstein's correction in fp64 (q = RN64(x y)
+// faithful, residual q d - x exact by FMA, no under/overflow for fp32
+// operands) gives RN64(x / d); rounding that to fp32 equals RN32(x / d)
+// (double rounding through 53 >= 2*24 + 2 bits is innocuous for division).
+// The residual is formed as q d - x and subtracted, so that x = -0 keeps its
+// sign (-0 + -0); infinite x takes q itself (the residual is inf - inf).
+// Both are checked against IEEE division over all 2^32 inputs by
+// lbm_check_division.  fp64 runs keep the IEEE division.
+__device__ __forceinline__ float divk(float x, float, double y) {
+  return (float)((double)x * y);
+}
+__device__ __forceinline__ float divk_any(float x, float d, double y) {
+  const double xd = (double)x, dd = (double)d;
+  const double q = xd * y;
+  const double e = __builtin_fma(q, dd, -xd);
+  const double r = __builtin_fma(-e, y, q);
+  return (float)(__builtin_isinf(xd) ? q : r);
 }
 __device__ __forceinline__ double divk(double x, double d, double) { return x / d; }
+__device__ __forceinline__ double divk_any(double x, double d, double) { return x / d; }
+// RN64(1/d) for the divk forms (unused by fp64 runs)
+__device__ __forceinline__ double rcp64(float d) { return 1.0 / (double)d; }
+__device__ __forceinline__ double rcp64(double) { return 0.0; }
 
 template <typename T> __device__ __forceinline__ T clamp01(T v) {
   return v < T(0) ? T(0) : (v > T(1) ? T(1) : v);
@@ -569,7 +590,7 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ partia
 // negatives of each other (same terms, negated, summed in the same order, and
 // round-to-nearest is symmetric), so only one of each pair is divided.
 template <typename T, int DIM>
-__device__ __forceinline__ void ediv(const Phys<T>& ph, const T* v, T d, T r,
+__device__ __forceinline__ void ediv(const Phys<T>& ph, const T* v, T d, double r,
                                      T (&out)[Lat<DIM>::Q]) {
   constexpr int Q = Lat<DIM>::Q;
   out[0] = T(0);
@@ -1084,7 +1105,7 @@ __device__ __forceinline__ void collide_node_src(const CollideArgs<T, S, DIM>& a
   {
     T gm = (T)sqrt(vsq<DIM>(gr));
     gm = clamp_lo(gm, T(1e-10));
-    const T lam = divk((T(4) * p) * (T(1) - p), ph.xi, ph.rcp_xi);
+    const T lam = divk_any((T(4) * p) * (T(1) - p), ph.xi, ph.rcp_xi);
     const T sf = ph.sfac[dist_class(m)];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -1124,12 +1145,20 @@ __device__ __forceinline__ void collide_node_src(const CollideArgs<T, S, DIM>& a
   // negatives), then g* and u are stored
   {
     const T rc = rho_eff * ph.cs2;
-    const T rrc = T(1) / rc;
+    const double rrc = rcp64(rc);
     T G[Q];
     G[0] = T(0);
+    // rc < 2 (densities below 6, the usual case) cannot be an even integer
+    if (ph.rc_small) {
 #pragma unroll
-    for (int i = 1; i < Q; ++i)
-      if (OPP<DIM>(i) > i) G[i] = divk((ns * ph.template w<DIM>(i)) * edot<DIM>(i, F), rc, rrc);
+      for (int i = 1; i < Q; ++i)
+        if (OPP<DIM>(i) > i) G[i] = divk((ns * ph.template w<DIM>(i)) * edot<DIM>(i, F), rc, rrc);
+    } else {
+#pragma unroll
+      for (int i = 1; i < Q; ++i)
+        if (OPP<DIM>(i) > i)
+          G[i] = divk_any((ns * ph.template w<DIM>(i)) * edot<DIM>(i, F), rc, rrc);
+    }
 #pragma unroll
     for (int i = 1; i < Q; ++i)
       if (OPP<DIM>(i) < i) G[i] = -G[OPP<DIM>(i)];

@@ -24,7 +24,8 @@ template <typename T> struct Phys {
   T lic[3];           // 2 * w                   (lbm/fe_ac_solver.py:293)
   T w3c[3];           // T(3 w), the bf16 g reference's weight
   T cs2, two_cs2;
-  T rcp_cs2, rcp_two_cs2, rcp_xi;   // RN(1/cs2), RN(1/(2 cs2)), RN(1/xi)
+  double rcp_cs2, rcp_two_cs2, rcp_xi;   // RN64(1/d) of the fp32 cs2, 2 cs2, xi
+  int rc_small;       // rho_eff cs2 < 2 for every density the run can form
   T four_beta, kappa, rho_l, rho_g, rho_floor, drho, visc, xi, ac_scale;
   T mu_wall_add;      // linear mode: mu_wall * mu_wall_scale
   T phi_c;            // quadratic mode

@@ -91,7 +91,11 @@ def main():
                 out[f"s{s}_{k}"] = np.array(v)
             out[f"s{s}_phi_xz"] = sol.phi.numpy()[:, 75, :].copy()
             out[f"s{s}_uz_xz"] = sol.u.numpy()[2, :, 75, :].copy()
-            np.savez_compressed(os.path.join(HERE, fname), **out)
+            # written whole, then renamed: a run cut short keeps the last
+            # complete checkpoint set
+            tmp = os.path.join(HERE, fname + ".part.npz")
+            np.savez_compressed(tmp, **out)
+            os.replace(tmp, os.path.join(HERE, fname))
             print(f"step {s}: {np.mean(times[-min(len(times), 20):]):.3f} s/step",
                   flush=True)
         if s == 25 and not f64:

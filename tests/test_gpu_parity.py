@@ -199,14 +199,17 @@ def test_graph_replay_matches_eager():
 
 
 def test_division_by_constants_is_ieee_exact():
-    """The kernels divide by cs2, 2 cs2, xi and rho_eff cs2 with a precomputed
-    reciprocal plus one FMA correction (IEEE division outside [2^-100, 2^100]);
-    over all 2^32 fp32 inputs this equals IEEE division."""
+    """The kernels divide by cs2, 2 cs2 and rho_eff cs2 as one fp64 product
+    with the divisor's fp64 reciprocal, and by xi with an fp64 Markstein
+    correction; over all 2^32 fp32 inputs both equal IEEE division (the
+    product form only for divisors that are not even integers, which is
+    checked for the general form alone)."""
     need_gpu()
     import ctypes
     from lbm import _native as nat
     # cs2, 2 cs2, xi values, and rho_eff * cs2 over its range [0.5/3, 1/3]
     for d in (np.float32(1.0 / 3.0), np.float32(2.0 / 3.0), np.float32(4.0), np.float32(5.0),
+              np.float32(6.0), np.float32(3.0), np.float32(1.0),
               np.float32(0.5) * np.float32(1.0 / 3.0), np.float32(0.7312) * np.float32(1.0 / 3.0),
               np.float32(0.9999) * np.float32(1.0 / 3.0)):
         bad = ctypes.c_ulonglong(0)
