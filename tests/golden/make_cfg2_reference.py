@@ -17,6 +17,9 @@ With REF_DTYPE=float64 the same run is made in the reference's fp64 form
 tests/golden/cfg2_ridge162_f64_summary.npz (slices kept in fp64); the
 distance between the two is the reference's own fp32 rounding envelope.
 
+REF_OUT=<dir> writes the summary there instead (copy it into tests/golden/
+once the run has reached its last checkpoint).
+
 Usage (build container only):
     python tests/golden/make_cfg2_reference.py [steps]
     REF_DTYPE=float64 python tests/golden/make_cfg2_reference.py [steps]
@@ -93,9 +96,10 @@ def main():
             out[f"s{s}_uz_xz"] = sol.u.numpy()[2, :, 75, :].copy()
             # written whole, then renamed: a run cut short keeps the last
             # complete checkpoint set
-            tmp = os.path.join(HERE, fname + ".part.npz")
+            dest = os.environ.get("REF_OUT", HERE)
+            tmp = os.path.join(dest, fname + ".part.npz")
             np.savez_compressed(tmp, **out)
-            os.replace(tmp, os.path.join(HERE, fname))
+            os.replace(tmp, os.path.join(dest, fname))
             print(f"step {s}: {np.mean(times[-min(len(times), 20):]):.3f} s/step",
                   flush=True)
         if s == 25 and not f64:
