@@ -391,3 +391,36 @@ def test_fused_tile_rejects_oversized_tiles():
         s.set_occupancy(6, 9)
     assert s.launch()[2:4] == (0, 0)
     s.step(3)
+
+
+@pytest.mark.parametrize("rho_l", [1.0, 8.0])
+def test_division_forms_match_oracle_bitwise(rho_l):
+    """rho_eff cs2 reaches 2 or more for liquid densities above 5.7, where the
+    kernels divide the Guo forcing terms by it with the general (Markstein)
+    form instead of the one-product form; both must reproduce the oracle's
+    IEEE fp32 divisions bit for bit (24x24x20 ridge impact, fp32, 3 steps)."""
+    need_gpu()
+    from geometry.substrate import create_substrate_with_fraction
+    from lbm.fe_ac_solver import AllenCahnSolver
+    from lbm.fe_config import FEConfig
+    from lbm.fe_droplet import create_fe_droplet_with_impact
+    print(f"[gpu] division forms, rho_l = {rho_l}", flush=True)
+    kw = dict(nx=24, ny=24, nz=20, rho_l=rho_l, rho_g=rho_l / 828.0, sigma=0.01424, xi=4.0,
+              tau_l=0.53, tau_g=0.53, theta_eq=162.0)
+    solid, frac = create_substrate_with_fraction(24, 24, 20, "ridge", 1.0, 6.0)
+    phi0, _, u0 = create_fe_droplet_with_impact(24, 24, 20, center=(12, 12, 13), radius=5.0,
+                                                xi=4.0, u_impact=(0, 0, -0.05))
+    s = AllenCahnSolver(FEConfig(**kw, device="cuda"), dtype=torch.float32,
+                        geo_amplification=1.5, geometric_wetting=True)
+    s.set_solid(solid, solid_fraction=frac)
+    s.init_fields(phi0, u0)
+    o = OracleSolver(Params(**kw), dtype=np.float32, geo_amplification=1.5,
+                     geometric_wetting=True)
+    o.set_solid(solid, frac)
+    o.init_fields(phi0, u0)
+    for step in range(1, 4):
+        s.step()
+        o.step()
+        for key in ("phi", "u", "P", "f", "g"):
+            np.testing.assert_array_equal(np_(getattr(s, key)), getattr(o, key),
+                                          err_msg=f"step {step} {key}")
