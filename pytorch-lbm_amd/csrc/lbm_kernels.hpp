@@ -317,7 +317,10 @@ __device__ __forceinline__ Nbrs<DIM> neighbours(const Grid& g, uint32_t x, const
 // LOCAL: the buffers hold post-stream values (first step after init_fields)
 // and are read in place.  It is a template parameter so that the common form
 // has no branch between the geometry-byte load and the pulls.
-template <typename T, typename S, int DIM, bool DEV, bool WITH_G, bool LOCAL, int CP = 0>
+// BB = false: the caller discards the values of solid nodes (pass A sets
+// their phi to 0), so their bounce-back loads are skipped.
+template <typename T, typename S, int DIM, bool DEV, bool WITH_G, bool LOCAL, int CP = 0,
+          bool BB = true>
 __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, const Nbrs<DIM>& nb,
                                      uint8_t m, const T* __restrict__ eps_arr,
                                      const Buf& pref, const Phys<T>& ph, T (&f)[Lat<DIM>::Q],
@@ -350,7 +353,7 @@ __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, con
   // nothing that reads m may be scheduled above the loads
   __builtin_amdgcn_sched_barrier(0);
   const bool solid = (m & M_SOLID) != 0;
-  if (solid) {
+  if (BB && solid) {
     T bf[Lat<DIM>::Q], bg[Lat<DIM>::Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -680,8 +683,9 @@ __device__ __forceinline__ T phi_node(const Grid& g, const Phys<T>& ph, const S*
   const Nbrs<DIM> nb = neighbours<DIM>(g, x, c);
   T f[Q], unused[Q];
   const Buf bF = dist_buf<DIM>(Fin, N), bP = field_buf(phi_prev, N);
-  pull<T, S, DIM, !std::is_same<T, S>::value, false, LOCAL>(bF, bF, x, nb, m, eps, bP, ph, f,
-                                                            unused);
+  // a solid node's phi is 0 whatever its distributions: no bounce-back loads
+  pull<T, S, DIM, !std::is_same<T, S>::value, false, LOCAL, 0, false>(bF, bF, x, nb, m, eps, bP,
+                                                                      ph, f, unused);
   if constexpr (!LOCAL)
     scale_f<T, DIM>(ph, f, cprev, m, (m & M_TOP) ? phi_prev[x] : T(0), nullptr,
                     nsw_phi<T>(ph, m, phi_prev, x));
