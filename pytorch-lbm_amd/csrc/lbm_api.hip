@@ -193,10 +193,10 @@ static int set_geometry_impl(lbm_solver* s, const void* frac, int fdt, hipStream
   if (s->nsw) k_nsw<DIM, BS><<<nb, BS, 0, st>>>(g, s->bnd, s->solid, s->normal, s->nsw);
   const long long ncol = DIM == 3 ? (long long)g.nx * g.ny : g.nx;
   k_surface<DIM, BS><<<nblk(ncol), BS, 0, st>>>(g, s->solid, s->surf);
-  // pass-B chunk lists: plain-fluid chunks, then the others (set-up only: one
-  // read-back of a byte per 64 nodes)
+  // pass-B chunk lists: plain-fluid chunks, then the others, all-solid ones
+  // tagged (kChunkSolid) (set-up only: one read-back of a byte per 64 nodes)
   const size_t nch = (size_t)((s->N + 63) / 64);
-  k_chunk_plain<BS><<<nblk((long long)nch * 64), BS, 0, st>>>(s->N, s->meta, s->chunk_flag);
+  k_chunk_kind<BS><<<nblk((long long)nch * 64), BS, 0, st>>>(s->N, s->meta, s->chunk_flag);
   HIPCHK(hipGetLastError());
   std::vector<uint8_t> flag(nch);
   HIPCHK(hipMemcpyAsync(flag.data(), s->chunk_flag, nch, hipMemcpyDeviceToHost, st));
@@ -204,10 +204,10 @@ static int set_geometry_impl(lbm_solver* s, const void* frac, int fdt, hipStream
   std::vector<uint32_t> list;
   list.reserve(nch);
   for (size_t c = 0; c < nch; ++c)
-    if (flag[c]) list.push_back((uint32_t)c);
+    if (flag[c] == 1) list.push_back((uint32_t)c);
   s->n_plain = (unsigned int)list.size();
   for (size_t c = 0; c < nch; ++c)
-    if (!flag[c]) list.push_back((uint32_t)c);
+    if (flag[c] != 1) list.push_back((uint32_t)c | (flag[c] == 2 ? kChunkSolid : 0u));
   s->n_wall = (unsigned int)(nch - s->n_plain);
   HIPCHK(hipMemcpyAsync(s->chunks, list.data(), nch * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));

@@ -136,7 +136,8 @@ __global__ __launch_bounds__(BS) void k_collide_batch_chunks(
   if (lb >= nb) return;
   const unsigned int w = lb * (BS / 64) + (threadIdx.x >> 6);
   if (w >= nch) return;
-  const uint32_t x = (cs.chunks[(FLUID ? 0u : cs.n_plain) + w] << 6) + (threadIdx.x & 63u);
+  const uint32_t ce = __builtin_amdgcn_readfirstlane(cs.chunks[(FLUID ? 0u : cs.n_plain) + w]);
+  const uint32_t x = ((ce & kChunkIdx) << 6) + (threadIdx.x & 63u);
   if (x >= N) return;
   CollideArgs<T, S, DIM> a;
   a.g = cs.g;
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(BS) void k_collide_batch_chunks(
   a.local = cs.local;
   a.phiu_valid = cs.phiu_valid;
   if (cs.local) collide_node<T, S, DIM, true, FLUID>(a, cs.ph, x);
-  else collide_node<T, S, DIM, false, FLUID>(a, cs.ph, x);
+  else collide_node<T, S, DIM, false, FLUID>(a, cs.ph, x, FLUID || !(ce & kChunkSolid));
 }
 
 // Pass B of every case in one launch over its single-launch chunk list (each
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(BS) void k_collide_batch_mixed(
   const unsigned int w = xcd_block(i, nb) * (BS / 64) + (threadIdx.x >> 6);
   const uint32_t c = __builtin_amdgcn_readfirstlane(cs.mixed[w]);
   if (c == 0xFFFFFFFFu) return;
-  const uint32_t x = ((c & 0x7FFFFFFFu) << 6) + (threadIdx.x & 63u);
+  const uint32_t x = ((c & kChunkIdx) << 6) + (threadIdx.x & 63u);
   if (x >= N) return;
   CollideArgs<T, S, DIM> a;
   a.g = cs.g;
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(BS) void k_collide_batch_mixed(
   a.local = cs.local;
   a.phiu_valid = cs.phiu_valid;
   if (cs.local) collide_node<T, S, DIM, true, false>(a, cs.ph, x);
-  else if (c >> 31) collide_node<T, S, DIM, false, false>(a, cs.ph, x);
+  else if (c & kChunkWall) collide_node<T, S, DIM, false, false>(a, cs.ph, x, !(c & kChunkSolid));
   else collide_node<T, S, DIM, false, true>(a, cs.ph, x);
 }
 

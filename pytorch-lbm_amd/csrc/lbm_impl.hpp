@@ -86,7 +86,7 @@ struct lbm_solver {
   // pass-B chunk lists (64-node chunks): plain-fluid chunks first, then the
   // rest; built at set_geometry
   uint32_t* chunks = nullptr;
-  uint8_t* chunk_flag = nullptr;
+  uint8_t* chunk_flag = nullptr;     // kind of every chunk (k_chunk_kind)
   unsigned int n_plain = 0, n_wall = 0;
   // the same chunks as one list per pass-B block size (128, 256, 512) for the
   // single-launch form: per XCD range, plain chunks then wall chunks

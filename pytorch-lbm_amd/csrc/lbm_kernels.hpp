@@ -35,6 +35,11 @@ namespace lbm {
 // Cache-policy operand of buffer loads and stores: the NT (non-temporal) bit.
 constexpr int kNT = 2;
 
+// Pass-B chunk-list entries: a 64-node chunk index plus two flags -- bit 31
+// marks a wall chunk in the single-launch list, bit 30 a chunk whose 64 nodes
+// are all solid (no fluid pull of it is ever used).
+constexpr uint32_t kChunkWall = 0x80000000u, kChunkSolid = 0x40000000u, kChunkIdx = 0x3FFFFFFFu;
+
 struct Ctrl {
   double M0;
   double mass[2];     // fp64 sum of the start-of-step phi, by step parity
@@ -319,12 +324,14 @@ __device__ __forceinline__ Nbrs<DIM> neighbours(const Grid& g, uint32_t x, const
 // has no branch between the geometry-byte load and the pulls.
 // BB = false: the caller discards the values of solid nodes (pass A sets
 // their phi to 0), so their bounce-back loads are skipped.
+// spec = false (wave-uniform): every node of the wave is solid, so the
+// speculative fluid pulls are not issued.
 template <typename T, typename S, int DIM, bool DEV, bool WITH_G, bool LOCAL, int CP = 0,
           bool BB = true>
 __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, const Nbrs<DIM>& nb,
                                      uint8_t m, const T* __restrict__ eps_arr,
                                      const Buf& pref, const Phys<T>& ph, T (&f)[Lat<DIM>::Q],
-                                     T (&gg)[Lat<DIM>::Q]) {
+                                     T (&gg)[Lat<DIM>::Q], bool spec = true) {
   constexpr int Q = Lat<DIM>::Q;
   const uint32_t vx = dvoff<Q, S>(x);
   if constexpr (LOCAL) {
@@ -344,11 +351,19 @@ __device__ __forceinline__ void pull(const Buf& F, const Buf& G, uint32_t x, con
   // known (one memory latency instead of two in series); solid lanes then
   // take bounce-back values instead.  The choice is a select after the
   // branch, so the compiler cannot sink the fluid loads behind the branch.
+  if (spec) {
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const uint32_t vs = dvoff<Q, S>(nb.i[OPP<DIM>(i)]);
-    f[i] = dload<Q, T, S, CP>(F, i, vs);
-    if constexpr (WITH_G) gg[i] = dload<Q, T, S, CP>(G, i, vs);
+    for (int i = 0; i < Q; ++i) {
+      const uint32_t vs = dvoff<Q, S>(nb.i[OPP<DIM>(i)]);
+      f[i] = dload<Q, T, S, CP>(F, i, vs);
+      if constexpr (WITH_G) gg[i] = dload<Q, T, S, CP>(G, i, vs);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      f[i] = T(0);
+      if constexpr (WITH_G) gg[i] = T(0);
+    }
   }
   // nothing that reads m may be scheduled above the loads
   __builtin_amdgcn_sched_barrier(0);
@@ -421,11 +436,11 @@ template <typename T, typename S, int DIM, bool GREF, bool LOCAL, int CP = 0>
 __device__ __forceinline__ void pull_g(const Buf& G, const Buf& bU, uint32_t N32, uint32_t x,
                                        const Nbrs<DIM>& nb, uint8_t m,
                                        const T* __restrict__ eps_arr, const Phys<T>& ph,
-                                       T (&g)[Lat<DIM>::Q]) {
+                                       T (&g)[Lat<DIM>::Q], bool spec = true) {
   constexpr int Q = Lat<DIM>::Q;
   if constexpr (!GREF) {
     T unused[Q];
-    pull<T, S, DIM, false, false, LOCAL, CP>(G, G, x, nb, m, eps_arr, G, ph, g, unused);
+    pull<T, S, DIM, false, false, LOCAL, CP>(G, G, x, nb, m, eps_arr, G, ph, g, unused, spec);
   } else {
     const uint32_t vx = dvoff<Q, S>(x);
     if constexpr (LOCAL) {
@@ -433,8 +448,13 @@ __device__ __forceinline__ void pull_g(const Buf& G, const Buf& bU, uint32_t N32
       for (int i = 0; i < Q; ++i) g[i] = dload<Q, T, S>(G, i, vx) + gref<T, DIM>(ph, i, bU, N32, x);
       return;
     }
+    if (spec) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) g[i] = dload<Q, T, S, CP>(G, i, dvoff<Q, S>(nb.i[OPP<DIM>(i)]));
+      for (int i = 0; i < Q; ++i) g[i] = dload<Q, T, S, CP>(G, i, dvoff<Q, S>(nb.i[OPP<DIM>(i)]));
+    } else {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) g[i] = T(0);
+    }
 #if LBM_GREF_EARLY
     // the sources' velocities are loaded together with the g slots (one
     // memory latency instead of two in series)
@@ -952,9 +972,12 @@ template <typename T, typename S, int DIM> struct CollideArgs {
 // top flag), so every wall path compiles away.  GMU: the interface source
 // uses grad mu (Cahn-Hilliard or the mu drive), from the mu of every node.
 // PS: source of the start-of-step phi (PhiGlobal or PhiRing).
+// spec = false: every node of the wave is solid (a flagged chunk), so the
+// speculative fluid pulls are skipped; the bounce-back collision is the same.
 template <typename T, typename S, int DIM, bool LOCAL, bool FLUID, bool GMU, typename PS>
 __device__ __forceinline__ void collide_node_src(const CollideArgs<T, S, DIM>& a,
-                                                 const Phys<T>& ph, uint32_t x, const PS& ps) {
+                                                 const Phys<T>& ph, uint32_t x, const PS& ps,
+                                                 bool spec = true) {
   constexpr int Q = Lat<DIM>::Q;
   constexpr bool DEV = !std::is_same<T, S>::value;
   const Grid& g = a.g;
@@ -990,7 +1013,7 @@ __device__ __forceinline__ void collide_node_src(const CollideArgs<T, S, DIM>& a
   // priority, so its loads leave early while other waves compute (measured
   // 142.1 -> 140.6 us/step for bf16; no gain for 4-byte storage)
   if constexpr (DEV) __builtin_amdgcn_s_setprio(1);
-  pull_g<T, S, DIM, kGRef<S>, LOCAL, CP_G>(bGi, bU, N32, x, nb, m, a.eps, ph, gg);
+  pull_g<T, S, DIM, kGRef<S>, LOCAL, CP_G>(bGi, bU, N32, x, nb, m, a.eps, ph, gg, spec);
   T f[Q];
 #if LBM_EARLY_F
   // the f pulls leave together with the g pulls: their latency hides behind
@@ -1189,7 +1212,8 @@ __device__ __forceinline__ void collide_node_src(const CollideArgs<T, S, DIM>& a
   asm volatile("" : "+v"(x2));
   const Nbrs<DIM> nb2 = neighbours<DIM>(g, x2, decompose<DIM>(g, x2));
   if constexpr (DEV) __builtin_amdgcn_s_setprio(1);
-  pull<T, S, DIM, DEV, false, LOCAL, CP_FLD>(bFi, bFi, x, nb2, m, a.eps, bPp, ph, f, unused);
+  pull<T, S, DIM, DEV, false, LOCAL, CP_FLD>(bFi, bFi, x, nb2, m, a.eps, bPp, ph, f, unused,
+                                              spec);
   if constexpr (DEV) __builtin_amdgcn_s_setprio(0);
 #endif
   if constexpr (!LOCAL)
@@ -1216,8 +1240,8 @@ __device__ __forceinline__ void collide_node_src(const CollideArgs<T, S, DIM>& a
 
 template <typename T, typename S, int DIM, bool LOCAL, bool FLUID = false, bool GMU = false>
 __device__ __forceinline__ void collide_node(const CollideArgs<T, S, DIM>& a,
-                                             const Phys<T>& ph, uint32_t x) {
-  collide_node_src<T, S, DIM, LOCAL, FLUID, GMU>(a, ph, x, PhiGlobal<T>(a.phi_cur, a.g.N));
+                                             const Phys<T>& ph, uint32_t x, bool spec = true) {
+  collide_node_src<T, S, DIM, LOCAL, FLUID, GMU>(a, ph, x, PhiGlobal<T>(a.phi_cur, a.g.N), spec);
 }
 
 // Occupancy target of pass B (waves per SIMD); a build-time tuning knob.
@@ -1248,9 +1272,12 @@ __global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_collide_chunks(
   if (finalize && blockIdx.x == 0) finalize_mass<BS>(a.partials, a.nparts, a.ctrl_w, a.slot_new);
   const unsigned int w = xcd_block(blockIdx.x, gridDim.x) * (BS / 64) + (threadIdx.x >> 6);
   if (w >= nchunks) return;
-  const uint32_t x = (chunks[w] << 6) + (threadIdx.x & 63u);
+  // one entry per wave: a scalar
+  const uint32_t c = __builtin_amdgcn_readfirstlane(chunks[w]);
+  const uint32_t x = ((c & kChunkIdx) << 6) + (threadIdx.x & 63u);
   if (x >= a.g.N) return;
-  collide_node<T, S, DIM, LOCAL, FLUID>(a, ph, x);
+  // plain chunks are never all solid: the flag is a constant there
+  collide_node<T, S, DIM, LOCAL, FLUID>(a, ph, x, FLUID || !(c & kChunkSolid));
 }
 
 // Pass B in one launch over a chunk list arranged on the host so that each
@@ -1269,9 +1296,9 @@ __global__ __launch_bounds__(BS) LBM_COLLIDE_ATTR void k_collide_mixed(
   // one entry per wave: a scalar, so the branch below is uniform
   const uint32_t c = __builtin_amdgcn_readfirstlane(list[w]);
   if (c == 0xFFFFFFFFu) return;
-  const uint32_t x = ((c & 0x7FFFFFFFu) << 6) + (threadIdx.x & 63u);
+  const uint32_t x = ((c & kChunkIdx) << 6) + (threadIdx.x & 63u);
   if (x >= a.g.N) return;
-  if (c >> 31) collide_node<T, S, DIM, LOCAL, false>(a, ph, x);
+  if (c & kChunkWall) collide_node<T, S, DIM, LOCAL, false>(a, ph, x, !(c & kChunkSolid));
   else collide_node<T, S, DIM, LOCAL, true>(a, ph, x);
 }
 
@@ -1399,14 +1426,19 @@ __global__ __launch_bounds__(BS) void k_fused_mass(const double* __restrict__ bp
 
 // Chunk classification: 1 when all 64 nodes of the chunk exist and are plain
 // fluid.
+// Kind of every 64-node chunk: 1 plain (every node fluid with no solid,
+// partial, wall or top flag), 2 all solid (every node solid, none beyond N),
+// 0 the rest.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_chunk_plain(long long N, const uint8_t* __restrict__ meta,
-                                                   uint8_t* __restrict__ plain) {
+__global__ __launch_bounds__(BS) void k_chunk_kind(long long N, const uint8_t* __restrict__ meta,
+                                                  uint8_t* __restrict__ kind) {
   const long long x = (long long)blockIdx.x * BS + threadIdx.x;
-  const bool ok = x < N && (meta[x] & 0x0Fu) == 0;
-  const unsigned long long all = __ballot(ok);
+  const bool in = x < N;
+  const uint8_t m = in ? meta[x] : (uint8_t)0;
+  const unsigned long long plain = __ballot(in && (m & 0x0Fu) == 0);
+  const unsigned long long solid = __ballot(in && (m & M_SOLID) != 0);
   if ((threadIdx.x & 63) == 0 && (x >> 6) < (N + 63) / 64)
-    plain[x >> 6] = all == ~0ull ? 1 : 0;
+    kind[x >> 6] = plain == ~0ull ? 1 : (solid == ~0ull ? 2 : 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -19,7 +19,8 @@ constexpr unsigned int kListXCD = 8;
 // [j*(G/8) + min(j, G%8), ...) of G/8 (+1 for j < G%8).  Each XCD's range
 // gets a contiguous slice of the plain chunks, then a slice of the wall
 // chunks (bit 31 set) in proportion to its capacity; padding entries are
-// 0xFFFFFFFF.  `list` holds the plain chunks, then the wall chunks.
+// 0xFFFFFFFF.  `list` holds the plain chunks, then the wall chunks; other
+// flag bits of its entries (bit 30: all nodes solid) are carried over.
 inline bool mixed_list(const std::vector<uint32_t>& list, unsigned int np, unsigned int nw,
                        unsigned int wpb, std::vector<uint32_t>& mx) {
   const unsigned int n = np + nw, G = (n + wpb - 1) / wpb;
