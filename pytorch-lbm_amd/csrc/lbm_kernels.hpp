@@ -1175,7 +1175,7 @@ __device__ __forceinline__ void collide_node_src(const CollideArgs<T, S, DIM>& a
     const double rrc = rcp64(rc);
     T G[Q];
     G[0] = T(0);
-    // rc < 2 (densities below 6, the usual case) cannot be an even integer
+    // rc < 2 (densities below 5.7, the usual case) cannot be an even integer
     if (ph.rc_small) {
 #pragma unroll
       for (int i = 1; i < Q; ++i)
