@@ -4,11 +4,16 @@ cfg2_ridge162_summary.npz, written by tests/golden/make_cfg2_reference.py):
 mid-plane phi and u_z slices, phi/u moments, extents and z_com at steps
 1, 10, 100, 500, 1000, 1500, 2000.
 
-The reference's fp32 trajectory is itself sensitive to rounding (a hard
-|grad phi| switch in the gradient correction acts on far-field noise), so
-after the first steps the gate is an envelope: the distance to the
-reference must stay within max(tol, 4x) of the distance between this
-build's fp32 and fp64 runs of the same kernels."""
+Through step 10 the gate is a flat 1e-5 (relative L2 of every compared
+quantity).  After that the reference's fp32 trajectory leaves its own fp64
+trajectory (a hard |grad phi| > 1e-6 switch in the gradient correction,
+lbm/fe_wetting.py:525, acts on far-field rounding noise), so the gate is an
+envelope taken from the reference alone: the distance to the reference's fp32
+run must stay within max(2e-5, 4x) the distance between the reference's fp32
+and fp64 runs (tests/golden/cfg2_ridge162_f64_summary.npz, the same script
+with float64) at that checkpoint.  This build's fp64 run of the same kernels
+is printed beside it (the error of the fp32 form against fp64) but gates
+nothing."""
 import os
 
 import numpy as np
@@ -62,32 +67,33 @@ def test_cfg2_2000_steps_against_reference_run():
     need_gpu()
     print("[gpu] cfg2 2000 steps vs reference summary", flush=True)
     ref = np.load(os.path.join(HERE, "golden", "cfg2_ridge162_summary.npz"))
+    r64 = np.load(os.path.join(HERE, "golden", "cfg2_ridge162_f64_summary.npz"))
     s32, solid = build(torch.float32)
     s64, _ = build(torch.float64)
     assert abs(s32.phi_mass_init - float(ref["M0"])) / float(ref["M0"]) < 1e-6
+    keys = ("phi_xz", "uz_xz", "phi_sum", "phi_sumsq", "u2_sum", "z_com")
     k = 0
     for cp in CPS:
         s32.step(cp - k)
         s64.step(cp - k)
         k = cp
         a, b = summary(s32, solid), summary(s64, solid)
+        assert f"s{cp}_phi_xz" in r64.files, f"no fp64 reference checkpoint {cp}"
+        # the reference's own fp32-vs-fp64 distance at this checkpoint: the
+        # largest over everything compared (one scalar moment can be close by
+        # accident after the fields have diverged)
+        env = max(rel(ref[f"s{cp}_{key}"], r64[f"s{cp}_{key}"]) for key in keys)
+        own = max(rel(a[key], b[key]) for key in keys)
         line = []
-        keys = ("phi_xz", "uz_xz", "phi_sum", "phi_sumsq", "u2_sum", "z_com")
-        # trajectory-level divergence of fp32 from fp64 at this checkpoint: the
-        # largest distance over everything compared (one scalar moment can be
-        # close by accident after the fields have diverged)
-        env = max(rel(b[key], a[key]) for key in keys)
         for key in keys:
             e = rel(a[key], ref[f"s{cp}_{key}"])
             line.append(f"{key} {e:.2e}")
-            tol = 1e-5 if cp <= 10 else 2e-5
-            if key == "uz_xz" and cp <= 10:
-                tol = 1e-4       # u_z is near zero over most of the slice
-            assert e <= max(tol, 4 * env), (cp, key, e, env)
-        line.append(f"env {env:.2e}")
+            tol = 1e-5 if cp <= 10 else max(2e-5, 4 * env)
+            assert e <= tol, (cp, key, e, env)
+        line.append(f"ref32-ref64 {env:.2e}, hip32-hip64 {own:.2e}")
         for key in ("Dx", "Dy", "Dz"):
-            d = abs(a[key] - float(ref[f"s{cp}_{key}"]))
-            denv = abs(b[key] - a[key])
-            line.append(f"{key} {a[key]:.0f}/{float(ref[f's{cp}_{key}']):.0f}")
-            assert d <= max(1.0, 4 * denv), (cp, key, a[key], float(ref[f"s{cp}_{key}"]))
+            r = float(ref[f"s{cp}_{key}"])
+            denv = abs(r - float(r64[f"s{cp}_{key}"]))
+            line.append(f"{key} {a[key]:.0f}/{r:.0f}")
+            assert abs(a[key] - r) <= max(1.0, 4 * denv), (cp, key, a[key], r)
         print(f"[gpu] step {cp}: " + ", ".join(line), flush=True)
