@@ -32,7 +32,7 @@ def specs():
     ]
 
 
-@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+@pytest.mark.parametrize("storage", ["fp32", "bf16", "fp16"])
 def test_same_alone_and_in_the_mix(storage):
     need_gpu()
     from lbm.batch import BatchEngine
@@ -134,7 +134,7 @@ def _solver_case(s, steps, storage=None, dtype=torch.float32):
     return sol
 
 
-@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+@pytest.mark.parametrize("storage", ["fp32", "bf16", "fp16"])
 def test_batch_case_matches_solver_api(storage):
     """A batch case built from the host generators equals the AllenCahnSolver
     case built from the same generators, bit for bit (same inputs, same
