@@ -26,7 +26,7 @@ def gpu(name, storage):
     cfg = FEConfig(**config_kwargs(m), device="cuda")
     dt = torch.float64 if storage == "fp64" else torch.float32
     s = AllenCahnSolver(cfg, dtype=dt, storage=None if storage in ("fp32", "fp64") else storage,
-                        **solver_kwargs(m))
+                        **solver_kwargs(m, z, "cuda"))
     s.set_solid(z["solid"], z["fraction"] if "fraction" in z.files else None)
     s.init_fields(z["phi_init"], z["u_init"])
     return s, z
@@ -35,7 +35,7 @@ def gpu(name, storage):
 def oracle(name, storage):
     z, m = load(name)
     o = OracleSolver(Params(**config_kwargs(m)), dtype=np.float32, storage=storage,
-                     **solver_kwargs(m))
+                     **solver_kwargs(m, z))
     o.set_solid(z["solid"], z["fraction"] if "fraction" in z.files else None)
     o.init_fields(z["phi_init"], z["u_init"])
     return o
@@ -48,7 +48,9 @@ def test_round_bf16_matches_torch():
 
 
 @pytest.mark.parametrize("storage", ["bf16", "fp16"])
-@pytest.mark.parametrize("name", ["ridge162_readme_f32", "flat162_2d_f32", "ridge120_f32"])
+@pytest.mark.parametrize("name", ["ridge162_readme_f32", "flat162_2d_f32", "ridge120_f32",
+                                  "ridge162_ch_f32", "ridge144_direct_f32", "concave126_f32",
+                                  "convex144_nofrac_f32", "ridge120_alphafield_f32"])
 def test_16bit_follows_oracle_model(name, storage):
     need_gpu()
     print(f"[gpu] {storage} {name}", flush=True)
